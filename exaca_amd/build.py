@@ -1,0 +1,82 @@
+"""Builds the native library ``libexaca_amd.so`` and the ``exaca-mi355x`` executable in-tree.
+
+HIP sources are compiled for gfx950 only with hipcc; host-only C++ (deck parser, initial state,
+output writers) with g++. Floating-point contraction is off everywhere and fp32 divide/sqrt are
+correctly rounded, so the kernels evaluate the reference's expressions operation for operation.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+CSRC = PKG / "csrc"
+OUT = PKG / "_build"
+LIB = PKG / "libexaca_amd.so"
+EXE = PKG / "exaca-mi355x"
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CXX = os.environ.get("CXX", "g++")
+
+HIP_FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-ffp-contract=off",
+    "-fhip-fp32-correctly-rounded-divide-sqrt",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall",
+]
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wextra", "-Wno-unused-parameter"]
+
+HIP_SOURCES = ["ca_kernels.hip", "capi.hip"]
+CXX_SOURCES = ["deck.cpp", "init.cpp", "output.cpp"]
+
+
+def _run(cmd: list[str], verbose: bool) -> None:
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError(f"build step failed ({r.returncode}): {' '.join(cmd)}")
+    noise = [ln for ln in r.stderr.splitlines() if "packed-fp32-ops" not in ln and ln.strip()]
+    if verbose and noise:
+        sys.stderr.write("\n".join(noise) + "\n")
+
+
+def build(verbose: bool = False) -> Path:
+    """Compiles every source (always; no up-to-date check) and links the library and CLI."""
+    OUT.mkdir(exist_ok=True)
+    objs = []
+    for src in CXX_SOURCES:
+        obj = OUT / (src + ".o")
+        _run([CXX, *CXX_FLAGS, "-c", str(CSRC / src), "-o", str(obj)], verbose)
+        objs.append(str(obj))
+    for src in HIP_SOURCES:
+        obj = OUT / (src + ".o")
+        _run([HIPCC, *HIP_FLAGS, "-c", str(CSRC / src), "-o", str(obj)], verbose)
+        objs.append(str(obj))
+    tmp = LIB.with_suffix(".so.tmp")
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-o", str(tmp), *objs, "-lpthread", "-ldl"], verbose)
+    os.replace(tmp, LIB)
+    _run([CXX, "-O2", "-std=c++17", "-o", str(EXE), str(CSRC / "main.cpp"), f"-L{PKG}", "-lexaca_amd",
+          "-Wl,-rpath,$ORIGIN"], verbose)
+    return LIB
+
+
+def build_oracle(verbose: bool = False) -> Path:
+    """Builds the C oracle (test infrastructure) with its own Makefile."""
+    oracle = PKG.parent / "oracle"
+    _run(["make", "-s", "-B", "-C", str(oracle)], verbose)
+    return oracle / "liboracle.so"
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    build_oracle(verbose=True)
+    print(f"built {LIB} and {EXE}")

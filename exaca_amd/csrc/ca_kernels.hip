@@ -1,0 +1,496 @@
+// ca_kernels.hip -- the CA time step as memory-bound CDNA4 kernels over a compacted frontier.
+//
+// One step (reference: src/runCA.cpp:291-311) is three launches on one stream:
+//   k_nucleate  activates the pre-sampled nuclei whose time has come (src/CAupdate.cpp:67-200)
+//   k_claim     for every frontier cell past its liquidus time: growth of its octahedron, scan of
+//               the 26 neighbours, and a claim on every liquid neighbour it reaches, written as
+//               atomicMin of (generation << 32 | capturer index) into win[]
+//   k_apply     the winner of each claimed cell builds the new decentred octahedron; a frontier
+//               cell whose liquid neighbours are all taken by lower-indexed cells turns solid;
+//               survivors and new cells are appended to the next frontier list
+// Claim-then-apply reproduces the reference's Serial backend exactly: there pass 2 visits active
+// cells in ascending index order and every capture is visible at once, so a liquid cell goes to
+// the lowest-indexed cell that reaches it, and cell j still "sees" a liquid neighbour only if no
+// lower-indexed cell took it (src/CAupdate.cpp:264-352, 722-725).
+//
+// Every floating-point expression keeps the reference's operation order and float/double types;
+// the library is compiled with -ffp-contract=off and correctly rounded fp32 divide/sqrt.
+#include <hip/hip_runtime.h>
+
+#include "ca_kernels.hpp"
+
+namespace exaca {
+namespace {
+
+__device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }  // std::min
+__device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }  // std::max
+
+__device__ __forceinline__ int orientation_of(int gid) { return (abs(gid) - 1) % kNumOrientations; }
+
+// Steps executed in (t, cycle]: the reference adds the undercooling change once per executed step
+// while a cell is liquid or active past its liquidus time (src/CAupdate.cpp:235-237); cycle jumps
+// (src/CAupdate.cpp:812-814) skip steps J+1 .. T-1.
+__device__ __forceinline__ int executed_since(int t, int cycle, const StepCtl *ctl, int njumps) {
+    int n = cycle - t;
+    for (int q = 0; q < njumps; q++) {
+        const int lo = max(ctl->jump_from[q] + 1, t + 1);
+        const int hi = min(ctl->jump_to[q] - 1, cycle);
+        if (hi >= lo) n -= hi - lo + 1;
+    }
+    return n;
+}
+
+struct Coord {
+    int x, y, z;
+};
+__device__ __forceinline__ Coord decode(uint32_t p, int X, int Y) {
+    const uint32_t XY = static_cast<uint32_t>(X) * static_cast<uint32_t>(Y);
+    Coord c;
+    c.z = static_cast<int>(p / XY);
+    const uint32_t rem = p - static_cast<uint32_t>(c.z) * XY;
+    c.x = static_cast<int>(rem / static_cast<uint32_t>(Y));
+    c.y = static_cast<int>(rem - static_cast<uint32_t>(c.x) * static_cast<uint32_t>(Y));
+    return c;
+}
+__device__ __forceinline__ uint32_t nb_index(uint32_t p, int l, int X, int Y) {
+    return static_cast<uint32_t>(static_cast<long long>(p) + (static_cast<long long>(kNbZ[l]) * X + kNbX[l]) * Y +
+                                 kNbY[l]);
+}
+// Neighbours inside the rank's slab and below the active-domain top; the reference's edge lists
+// (src/CAupdate.cpp:276-318) and its MyNeighborZ < nzActive test (:328). z >= 0 always holds for
+// an active cell (plane 0 is a wall); it is tested for memory safety.
+__device__ __forceinline__ uint32_t valid_neighbours(const Coord &c, int X, int Y, int nzA) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int l = 0; l < 26; l++) {
+        const int x = c.x + kNbX[l], y = c.y + kNbY[l], z = c.z + kNbZ[l];
+        if (x >= 0 && x < X && y >= 0 && y < Y && z >= 0 && z < nzA) m |= 1u << l;
+    }
+    return m;
+}
+
+// Face normals of the octahedron as the reference forms them: float sums of unit-vector
+// components (src/CAupdate.cpp:526-551).
+struct Faces {
+    double fx[4], fy[4], fz[4];
+};
+__device__ __forceinline__ Faces faces_of(const float *g) {
+    Faces f;
+    f.fx[0] = g[0] + g[3] + g[6];
+    f.fx[1] = g[0] - g[3] + g[6];
+    f.fx[2] = g[0] + g[3] - g[6];
+    f.fx[3] = g[0] - g[3] - g[6];
+    f.fy[0] = g[1] + g[4] + g[7];
+    f.fy[1] = g[1] - g[4] + g[7];
+    f.fy[2] = g[1] + g[4] - g[7];
+    f.fy[3] = g[1] - g[4] - g[7];
+    f.fz[0] = g[2] + g[5] + g[8];
+    f.fz[1] = g[2] - g[5] + g[8];
+    f.fz[2] = g[2] + g[5] - g[8];
+    f.fz[3] = g[2] - g[5] - g[8];
+    return f;
+}
+// Critical diagonal length of neighbour l for an octahedron centred at c around the cell centre
+// (xp, yp, zp): the largest |F_q . (x_l - c)| (src/CAupdate.cpp:553-563), rounded to float as
+// the reference stores it.
+__device__ __forceinline__ float critical_length(const Faces &f, double xp, double yp, double zp, double cx,
+                                                 double cy, double cz, int l) {
+    const double ex = xp + kNbX[l] - cx, ey = yp + kNbY[l] - cy, ez = zp + kNbZ[l] - cz;
+    const double d0 = ex * f.fx[0] + ey * f.fy[0] + ez * f.fz[0];
+    const double d1 = ex * f.fx[1] + ey * f.fy[1] + ez * f.fz[1];
+    const double d2 = ex * f.fx[2] + ey * f.fy[2] + ez * f.fz[2];
+    const double d3 = ex * f.fx[3] + ey * f.fy[3] + ez * f.fz[3];
+    return static_cast<float>(dmax(dmax(fabs(d0), fabs(d1)), dmax(fabs(d2), fabs(d3))));
+}
+
+struct Octahedron {
+    double cx, cy, cz, dl;
+};
+__device__ __forceinline__ double sel3(int i, double a, double b, double c) { return i == 0 ? a : (i == 1 ? b : c); }
+
+// The captured cell's decentred octahedron (src/CAupdate.cpp:360-515): the capturing face is
+// found from the signs of the unit-vector projections, the face corner nearest the new cell
+// centre is the new octahedron's tip, and its half-diagonal follows from the projections of the
+// centre onto the two face edges through that corner.
+__device__ Octahedron new_octahedron(const float *g, double cxold, double cyold, double czold, double xp, double yp,
+                                     double zp) {
+    const double x0 = xp - cxold, y0 = yp - cyold, z0 = zp - czold;
+    const double mag0 = sqrtf(static_cast<float>(x0 * x0 + y0 * y0 + z0 * z0));
+    const double a1 = (g[0] * x0 + g[1] * y0 + g[2] * z0) / mag0;
+    const double a2 = (g[3] * x0 + g[4] * y0 + g[5] * z0) / mag0;
+    const double a3 = (g[6] * x0 + g[7] * y0 + g[8] * z0) / mag0;
+    const float s1 = (a1 < 0) ? 1.0f : -1.0f, s2 = (a2 < 0) ? 1.0f : -1.0f, s3 = (a3 < 0) ? 1.0f : -1.0f;
+    const double d1x = g[0] * s1, d1y = g[1] * s1, d1z = g[2] * s1;
+    const double d2x = g[3] * s2, d2y = g[4] * s2, d2z = g[5] * s2;
+    const double d3x = g[6] * s3, d3y = g[7] * s3, d3z = g[8] * s3;
+    const double u10 = d2x - d1x, u11 = d2y - d1y, u12 = d2z - d1z;
+    const double u20 = d3x - d1x, u21 = d3y - d1y, u22 = d3z - d1z;
+    const double uu0 = u11 * u22 - u12 * u21, uu1 = u12 * u20 - u10 * u22, uu2 = u10 * u21 - u11 * u20;
+    const double nd = sqrtf(static_cast<float>(uu0 * uu0 + uu1 * uu1 + uu2 * uu2));
+    const double n0 = uu0 / nd, n1 = uu1 / nd, n2 = uu2 / nd;
+    const double t = (n0 * x0 + n1 * y0 + n2 * z0) / (n0 * d1x + n1 * d1y + n2 * d1z);
+    const double tx0 = cxold + t * d1x, ty0 = cyold + t * d1y, tz0 = czold + t * d1z;
+    const double tx1 = cxold + t * d2x, ty1 = cyold + t * d2y, tz1 = czold + t * d2z;
+    const double tx2 = cxold + t * d3x, ty2 = cyold + t * d3y, tz2 = czold + t * d3z;
+    const double dc0 = sqrtf(static_cast<float>(((tx0 - xp) * (tx0 - xp)) + ((ty0 - yp) * (ty0 - yp)) +
+                                                ((tz0 - zp) * (tz0 - zp))));
+    const double dc1 = sqrtf(static_cast<float>(((tx1 - xp) * (tx1 - xp)) + ((ty1 - yp) * (ty1 - yp)) +
+                                                ((tz1 - zp) * (tz1 - zp))));
+    const double dc2 = sqrtf(static_cast<float>(((tx2 - xp) * (tx2 - xp)) + ((ty2 - yp) * (ty2 - yp)) +
+                                                ((tz2 - zp) * (tz2 - zp))));
+    const int bx = dc0 < dc1, by = dc1 < dc2, bz = dc2 < dc0;
+    const int idx = 2 * (bz - by) * bz + (by - bx) * by;
+    const int i1 = (idx + 1) % 3, i2 = (idx + 2) % 3;
+    const double mind = sel3(idx, dc0, dc1, dc2);
+    const double xc = sel3(idx, tx0, tx1, tx2), yc = sel3(idx, ty0, ty1, ty2), zc = sel3(idx, tz0, tz1, tz2);
+    const double x1 = sel3(i1, tx0, tx1, tx2), y1 = sel3(i1, ty0, ty1, ty2), z1 = sel3(i1, tz0, tz1, tz2);
+    const double x2 = sel3(i2, tx0, tx1, tx2), y2 = sel3(i2, ty0, ty1, ty2), z2 = sel3(i2, tz0, tz1, tz2);
+    const double D1 =
+        sqrtf(static_cast<float>(((xp - x2) * (xp - x2)) + ((yp - y2) * (yp - y2)) + ((zp - z2) * (zp - z2))));
+    const double D2 =
+        sqrtf(static_cast<float>(((xc - x2) * (xc - x2)) + ((yc - y2) * (yc - y2)) + ((zc - z2) * (zc - z2))));
+    const double D3 =
+        sqrtf(static_cast<float>(((xp - x1) * (xp - x1)) + ((yp - y1) * (yp - y1)) + ((zp - z1) * (zp - z1))));
+    const double D4 =
+        sqrtf(static_cast<float>(((xc - x1) * (xc - x1)) + ((yc - y1) * (yc - y1)) + ((zc - z1) * (zc - z1))));
+    double I1 = 0, I2 = D2, J1 = 0, J2 = D4;
+    if (mind != 0) {
+        I1 = D1 * ((xp - x2) * (xc - x2) + (yp - y2) * (yc - y2) + (zp - z2) * (zc - z2)) / (D1 * D2);
+        I2 = D2 - I1;
+        J1 = D3 * ((xp - x1) * (xc - x1) + (yp - y1) * (yc - y1) + (zp - z1) * (zc - z1)) / (D3 * D4);
+        J2 = D4 - J1;
+    }
+    const double sqrt3 = sqrt(3.0);
+    const double L12 = 0.5 * (dmin(I1, sqrt3) + dmin(I2, sqrt3));
+    const double L13 = 0.5 * (dmin(J1, sqrt3) + dmin(J2, sqrt3));
+    const double newdl = sqrt(2.0) * dmax(L12, L13);
+    const double c0 = xc - cxold, c1 = yc - cyold, c2 = zc - czold;
+    const double cn = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+    Octahedron o;
+    o.cx = xc - newdl * (c0 / cn);
+    o.cy = yc - newdl * (c1 / cn);
+    o.cz = zc - newdl * (c2 / cn);
+    o.dl = newdl;
+    return o;
+}
+
+// Wave64 inclusive prefix sum of v; returns the inclusive sum and sets total to the wave sum.
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v, uint32_t &total) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t n = __shfl_up(v, o, 64);
+        if (lane >= o) v += n;
+    }
+    total = __shfl(v, 63, 64);
+    return v;
+}
+
+__device__ __forceinline__ unsigned long long block_sum(unsigned long long v, unsigned long long *sh) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __syncthreads();
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    unsigned long long t = 0;
+    if (threadIdx.x == 0)
+        for (int q = 0; q < kBlock / 64; q++) t += sh[q];
+    return t;
+}
+
+// Contiguous slice of the frontier for this workgroup. Workgroups are dispatched round-robin over
+// the 8 XCDs, so consecutive slices are given to workgroups on the same XCD: neighbouring frontier
+// cells (and their neighbour reads and claims) then share one L2.
+__device__ __forceinline__ void frontier_slice(uint32_t count, uint32_t cap, uint32_t &begin, uint32_t &end) {
+    const uint32_t n = count < cap ? count : cap;  // an overflowed list is flagged, never over-read
+    const uint32_t nb = gridDim.x, b = blockIdx.x;
+    const uint32_t logical = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
+    uint32_t chunk = (n + nb - 1) / nb;
+    chunk = (chunk + kBlock - 1) / kBlock * kBlock;
+    const unsigned long long b0 = static_cast<unsigned long long>(logical) * chunk;
+    begin = b0 >= n ? n : static_cast<uint32_t>(b0);
+    end = (b0 + chunk >= n) ? n : static_cast<uint32_t>(b0 + chunk);
+}
+
+__global__ __launch_bounds__(kBlock) void k_nucleate(DomainView d, int p, int layer) {
+    StepCtl *ctl = d.ctl;
+    const int cycle = ctl->cycle[p];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->cycle[1 - p] = cycle + 1;
+        ctl->serial[1 - p] = ctl->serial[p] + 1;
+        ctl->count[1 - p] = 0;
+    }
+    unsigned long long nn = 0;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < d.nnuclei; i += gridDim.x * kBlock) {
+        const uint32_t q = static_cast<uint32_t>(d.nloc[i]);
+        if (!(cycle >= d.ntime[i] && d.ct[q] == kLiquid && d.lid[q] <= layer)) continue;
+        const Coord c = decode(q, d.X, d.Y);
+        if (c.x > 0 && c.x < d.X - 1 && c.y > 0 && c.y < d.Y - 1) nn++;
+        d.dl[q] = 0.01f;
+        d.cx[q] = (c.x + d.xoff) + 0.5;
+        d.cy[q] = (c.y + d.yoff) + 0.5;
+        d.cz[q] = c.z + 0.5;
+        d.ct[q] = kActive | kSeedBit;
+        const uint32_t slot = atomicAdd(&ctl->count[p], 1u);
+        if (slot < d.list_cap)
+            d.list[p][slot] = q;
+        else
+            atomicOr(&ctl->error, 1);
+    }
+    if (nn) atomicAdd(&d.stats->nucleated, nn);
+}
+
+__global__ __launch_bounds__(kBlock) void k_claim(DomainView d, int p, int layer) {
+    __shared__ unsigned long long sh[kBlock / 64];
+    const StepCtl *ctl = d.ctl;
+    const int cycle = ctl->cycle[p];
+    const unsigned long long stamp = static_cast<unsigned long long>(0xFFFFFFFFu - ctl->serial[p]) << 32;
+    const int njumps = ctl->njumps;
+    uint32_t begin, end;
+    frontier_slice(ctl->count[p], d.list_cap, begin, end);
+    const uint32_t *list = d.list[p];
+    unsigned long long front = 0;
+    for (uint32_t i = begin + threadIdx.x; i < end; i += kBlock) {
+        const uint32_t j = list[i];
+        const int t = d.cts[j];
+        if (!(cycle > t && d.lid[j] <= layer)) {
+            d.mask[i] = kWaitMask;
+            continue;
+        }
+        front++;
+        int k = executed_since(t, cycle, ctl, njumps);
+        k = k < 0 ? 0 : (k < d.ngrow - 1 ? k : d.ngrow - 1);
+        const float dlj = static_cast<float>(static_cast<double>(d.dl[j]) + d.grow[k]);
+        d.dl[j] = dlj;
+        const Coord c = decode(j, d.X, d.Y);
+        const uint32_t valid = valid_neighbours(c, d.X, d.Y, d.nzA);
+        uint32_t liquid = 0;
+#pragma unroll
+        for (int l = 0; l < 26; l++)
+            if ((valid >> l) & 1u)
+                if (d.ct[nb_index(j, l, d.X, d.Y)] == kLiquid) liquid |= 1u << l;
+        d.mask[i] = liquid;
+        if (!liquid) continue;
+        int o = orientation_of(d.gid[j]);
+        if (o < 0) {
+            atomicOr(&d.ctl->error, 2);
+            o = 0;
+        }
+        const unsigned long long key = stamp | j;
+        if (d.ct[j] & kSeedBit) {
+            const float *tab = d.seed_cdl + 26 * o;
+            for (uint32_t m = liquid; m; m &= m - 1) {
+                const int l = __builtin_ctz(m);
+                if (dlj >= tab[l]) atomicMin(&d.win[nb_index(j, l, d.X, d.Y)], key);
+            }
+        }
+        else {
+            const Faces f = faces_of(d.guv + 9 * o);
+            const double xp = (c.x + d.xoff) + 0.5, yp = (c.y + d.yoff) + 0.5, zp = c.z + 0.5;
+            const double cx = d.cx[j], cy = d.cy[j], cz = d.cz[j];
+            for (uint32_t m = liquid; m; m &= m - 1) {
+                const int l = __builtin_ctz(m);
+                if (dlj >= critical_length(f, xp, yp, zp, cx, cy, cz, l))
+                    atomicMin(&d.win[nb_index(j, l, d.X, d.Y)], key);
+            }
+        }
+    }
+    const unsigned long long tot = block_sum(front, sh);
+    if (threadIdx.x == 0 && tot) atomicAdd(&d.stats->frontier, tot);
+}
+
+__global__ __launch_bounds__(kBlock) void k_apply(DomainView d, int p, int layer) {
+    __shared__ unsigned long long sh[kBlock / 64];
+    StepCtl *ctl = d.ctl;
+    const unsigned long long stamp = static_cast<unsigned long long>(0xFFFFFFFFu - ctl->serial[p]) << 32;
+    uint32_t begin, end;
+    frontier_slice(ctl->count[p], d.list_cap, begin, end);
+    const uint32_t *list = d.list[p];
+    uint32_t *next = d.list[1 - p];
+    unsigned long long ncap = 0, nsol = 0, nwait = 0;
+    // All lanes of a wave run the same number of iterations so the wave-wide scan stays converged.
+    const uint32_t wave_begin = begin + (threadIdx.x & ~63u);
+    for (uint32_t base = wave_begin; base < end; base += kBlock) {
+        const uint32_t i = base + (threadIdx.x & 63u);
+        uint32_t j = 0, captured = 0, keep = 0;
+        if (i < end) {
+            j = list[i];
+            const uint32_t m = d.mask[i];
+            if (m == kWaitMask) {
+                keep = 1;
+                nwait++;
+            }
+            else {
+                uint32_t lcount = 0;
+                for (uint32_t r = m; r; r &= r - 1) {
+                    const int l = __builtin_ctz(r);
+                    const unsigned long long w = d.win[nb_index(j, l, d.X, d.Y)];
+                    if ((w & 0xFFFFFFFF00000000ull) == stamp) {
+                        const uint32_t wj = static_cast<uint32_t>(w);
+                        lcount |= (wj >= j);
+                        if (wj == j) captured |= 1u << l;
+                    }
+                    else {
+                        lcount = 1;
+                    }
+                }
+                keep = lcount;
+                if (!lcount) {
+                    d.ct[j] = kSolid;
+                    nsol++;
+                }
+            }
+        }
+        const uint32_t mine = keep + __builtin_popcount(captured);
+        uint32_t total;
+        const uint32_t incl = wave_inclusive_scan(mine, total);
+        uint32_t wbase = 0;
+        if ((threadIdx.x & 63) == 63 && total) wbase = atomicAdd(&ctl->count[1 - p], total);
+        wbase = __shfl(wbase, 63, 64);
+        if (static_cast<unsigned long long>(wbase) + total > d.list_cap) {
+            if ((threadIdx.x & 63) == 0) atomicOr(&ctl->error, 1);
+            continue;
+        }
+        uint32_t pos = wbase + incl - mine;
+        if (keep) next[pos++] = j;
+        if (captured) {
+            const int h = d.gid[j];
+            int o = orientation_of(h);
+            if (o < 0) {
+                atomicOr(&ctl->error, 2);
+                o = 0;
+            }
+            const float *g = d.guv + 9 * o;
+            const Coord c = decode(j, d.X, d.Y);
+            const double cxold = static_cast<float>(d.cx[j]);
+            const double cyold = static_cast<float>(d.cy[j]);
+            const double czold = static_cast<float>(d.cz[j]);
+            for (uint32_t r = captured; r; r &= r - 1) {
+                const int l = __builtin_ctz(r);
+                const uint32_t q = nb_index(j, l, d.X, d.Y);
+                const double xp = (c.x + d.xoff + kNbX[l]) + 0.5;
+                const double yp = (c.y + d.yoff + kNbY[l]) + 0.5;
+                const double zp = (c.z + kNbZ[l]) + 0.5;
+                const Octahedron oc = new_octahedron(g, cxold, cyold, czold, xp, yp, zp);
+                d.gid[q] = h;
+                d.dl[q] = static_cast<float>(oc.dl);
+                d.cx[q] = oc.cx;
+                d.cy[q] = oc.cy;
+                d.cz[q] = oc.cz;
+                d.ct[q] = kActive;
+                next[pos++] = q;
+                ncap++;
+            }
+        }
+    }
+    unsigned long long t = block_sum(ncap, sh);
+    if (threadIdx.x == 0 && t) atomicAdd(&d.stats->captures, t);
+    t = block_sum(nsol, sh);
+    if (threadIdx.x == 0 && t) atomicAdd(&d.stats->solidified, t);
+    t = block_sum(nwait, sh);
+    if (threadIdx.x == 0 && t) atomicAdd(&d.stats->waiting, t);
+}
+
+__global__ __launch_bounds__(kBlock) void k_check(DomainView d, int cycle, int layer, CheckResult *out) {
+    __shared__ unsigned long long sh[kBlock / 64];
+    __shared__ unsigned long long shmin[kBlock / 64];
+    unsigned long long c[5] = {0, 0, 0, 0, 0};
+    unsigned long long next = ~0ull;
+    const long long active_end = static_cast<long long>(d.X) * d.Y * d.nzA;
+    for (long long q = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x; q < d.ncell;
+         q += static_cast<long long>(gridDim.x) * kBlock) {
+        const int t = d.ct[q] & kTypeMask;
+        if (d.lid[q] == layer) {
+            if (t == kLiquid) {
+                const int crit = d.cts[q];
+                if (crit > cycle)
+                    c[0]++;
+                else
+                    c[1]++;
+                const unsigned long long v = static_cast<unsigned long long>(static_cast<long long>(crit));
+                if (q < active_end && v < next) next = v;
+            }
+            else if (t == kActive)
+                c[2]++;
+            else if (t == kSolid)
+                c[3]++;
+        }
+        else if (t == kLiquid) {
+            c[4]++;
+        }
+    }
+    for (int k = 0; k < 5; k++) {
+        const unsigned long long s = block_sum(c[k], sh);
+        if (threadIdx.x == 0 && s) atomicAdd(&out->counts[k], s);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long v = __shfl_down(next, o, 64);
+        next = v < next ? v : next;
+    }
+    if ((threadIdx.x & 63) == 0) shmin[threadIdx.x >> 6] = next;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long m = shmin[0];
+        for (int q = 1; q < kBlock / 64; q++) m = shmin[q] < m ? shmin[q] : m;
+        if (m != ~0ull) atomicMin(&out->next_cts, m);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_seed_init(DomainView d, const uint32_t *cells, uint32_t n) {
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const uint32_t q = cells[i];
+        const Coord c = decode(q, d.X, d.Y);
+        d.dl[q] = 0.01f;
+        d.cx[q] = (c.x + d.xoff) + 0.5;
+        d.cy[q] = (c.y + d.yoff) + 0.5;
+        d.cz[q] = c.z + 0.5;
+    }
+}
+
+// in: n rows of (orientation, cxold, cyold, czold, xp, yp, zp); out: n rows of
+// (cx, cy, cz, float(dl), 26 critical lengths recomputed from the new centre).
+__global__ __launch_bounds__(kBlock) void k_geometry_probe(const float *guv, const double *in, double *out, int n) {
+    const int r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= n) return;
+    const double *a = in + 7 * r;
+    const float *g = guv + 9 * static_cast<int>(a[0]);
+    const Octahedron o = new_octahedron(g, a[1], a[2], a[3], a[4], a[5], a[6]);
+    double *b = out + 30 * r;
+    b[0] = o.cx;
+    b[1] = o.cy;
+    b[2] = o.cz;
+    b[3] = static_cast<float>(o.dl);
+    const Faces f = faces_of(g);
+    for (int l = 0; l < 26; l++) b[4 + l] = critical_length(f, a[4], a[5], a[6], o.cx, o.cy, o.cz, l);
+}
+
+inline int grid_for(long long n, int cap) {
+    long long b = (n + kBlock - 1) / kBlock;
+    if (b < 1) b = 1;
+    return static_cast<int>(b < cap ? b : cap);
+}
+
+}  // namespace
+
+void launch_nucleation(const DomainView &d, int parity, int layer, hipStream_t s) {
+    k_nucleate<<<grid_for(d.nnuclei, 1024), kBlock, 0, s>>>(d, parity, layer);
+}
+void launch_claim(const DomainView &d, int parity, int layer, int blocks, hipStream_t s) {
+    k_claim<<<blocks, kBlock, 0, s>>>(d, parity, layer);
+}
+void launch_apply(const DomainView &d, int parity, int layer, int blocks, hipStream_t s) {
+    k_apply<<<blocks, kBlock, 0, s>>>(d, parity, layer);
+}
+void launch_check(const DomainView &d, int cycle, int layer, CheckResult *out, hipStream_t s) {
+    k_check<<<grid_for(d.ncell, 4096), kBlock, 0, s>>>(d, cycle, layer, out);
+}
+void launch_seed_init(const DomainView &d, const uint32_t *cells, uint32_t n, hipStream_t s) {
+    k_seed_init<<<grid_for(n, 1024), kBlock, 0, s>>>(d, cells, n);
+}
+void launch_geometry_probe(const float *guv, const double *in, double *out, int n, hipStream_t s) {
+    k_geometry_probe<<<grid_for(n, 1 << 20), kBlock, 0, s>>>(guv, in, out, n);
+}
+
+}  // namespace exaca

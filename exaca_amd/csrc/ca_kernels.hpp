@@ -1,0 +1,25 @@
+// ca_kernels.hpp -- host-side launchers for the CDNA4 kernels of one CA time step.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "ca_common.hpp"
+
+namespace exaca {
+
+constexpr int kBlock = 256;  // four wave64s per workgroup
+
+// Nucleation (src/CAupdate.cpp:67-200). Also prepares the control slot of the next step.
+void launch_nucleation(const DomainView &d, int parity, int layer, hipStream_t s);
+// CellCapture pass A: liquidus-time gate, growth, neighbour scan, claims (src/CAupdate.cpp:217-352).
+void launch_claim(const DomainView &d, int parity, int layer, int blocks, hipStream_t s);
+// CellCapture pass B: winners build new octahedra, losers/finished cells solidify, next frontier
+// (src/CAupdate.cpp:353-725).
+void launch_apply(const DomainView &d, int parity, int layer, int blocks, hipStream_t s);
+// IntermediateOutputAndCheck's reduction (src/CAupdate.cpp:738-758, 792-808).
+void launch_check(const DomainView &d, int cycle, int layer, CheckResult *out, hipStream_t s);
+// Octahedra of the initially active cells (src/CAinitialize.cpp:1936-2043).
+void launch_seed_init(const DomainView &d, const uint32_t *cells, uint32_t n, hipStream_t s);
+// Device-side self test of the capture geometry against host values: one octahedron per input row.
+void launch_geometry_probe(const float *guv, const double *in, double *out, int n, hipStream_t s);
+
+}  // namespace exaca

@@ -1,0 +1,579 @@
+// capi.hip -- the C ABI (include/exaca_ca.h): device state of one CA domain, the per-step entry
+// points, the device-resident layer loop replayed from HIP graphs, and the RunProgram_Reduced
+// drop-in.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdlib>
+#include <memory>
+#include <cstdio>
+#include <cstring>
+#include <iostream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/exaca_ca.h"
+#include "ca_common.hpp"
+#include "ca_kernels.hpp"
+#include "deck.hpp"
+#include "init.hpp"
+#include "output.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct DeviceError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct StateError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void hip_check(hipError_t e, const char *what) {
+    if (e != hipSuccess) throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HIP_OK(x) hip_check((x), #x)
+
+template <typename F>
+int guarded(F &&f) {
+    try {
+        f();
+        return EXACA_OK;
+    }
+    catch (const DeviceError &e) {
+        g_last_error = e.what();
+        return EXACA_ERR_DEVICE;
+    }
+    catch (const StateError &e) {
+        g_last_error = e.what();
+        return EXACA_ERR_STATE;
+    }
+    catch (const std::exception &e) {
+        g_last_error = e.what();
+        return EXACA_ERR_INPUT;
+    }
+    catch (...) {
+        g_last_error = "unknown error";
+        return EXACA_ERR_INPUT;
+    }
+}
+
+constexpr int kGraphSteps = 50;  // even: a graph starts and ends on parity 0
+
+}  // namespace
+
+struct exaca_state {
+    exaca::Deck deck;
+    exaca::HostDomain host;
+    exaca::DomainView dv{};
+    hipStream_t stream = nullptr;
+    exaca::StepCtl *ctl = nullptr;
+    exaca::StepStats *stats = nullptr;
+    exaca::CheckResult *check = nullptr;
+    std::vector<void *> buffers;
+    size_t bytes = 0;
+    int parity = 0;        // parity of the next step
+    int cycle = 0;         // last executed step
+    int pending = -1;      // cycle whose nucleation ran but whose capture has not
+    int blocks = 2048;     // workgroups of the frontier kernels
+    int njumps = 0;
+    hipGraphExec_t graph = nullptr;
+    int graph_layer = -1;
+    double init_seconds = 0;
+
+    template <typename T>
+    T *alloc(size_t n) {
+        void *p = nullptr;
+        HIP_OK(hipMalloc(&p, n * sizeof(T) > 0 ? n * sizeof(T) : 1));
+        buffers.push_back(p);
+        bytes += n * sizeof(T);
+        return static_cast<T *>(p);
+    }
+    ~exaca_state() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        for (void *p : buffers) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+void build_device_state(exaca_state *h) {
+    using namespace exaca;
+    const HostDomain &hd = h->host;
+    HIP_OK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIP_OK(hipGetDeviceProperties(&prop, dev));
+    h->blocks = ((prop.multiProcessorCount * 8 + 7) / 8) * 8;
+
+    DomainView &d = h->dv;
+    d.X = hd.X;
+    d.Y = hd.Y;
+    d.nz = hd.nz;
+    d.nzA = hd.nzA;
+    d.xoff = hd.xoff;
+    d.yoff = hd.yoff;
+    d.ncell = hd.ncell;
+    const size_t n = static_cast<size_t>(hd.ncell);
+    d.ct = h->alloc<uint8_t>(n);
+    d.gid = h->alloc<int32_t>(n);
+    d.cts = h->alloc<int32_t>(n);
+    d.lid = h->alloc<int8_t>(n);
+    d.dl = h->alloc<float>(n);
+    d.cx = h->alloc<double>(n);
+    d.cy = h->alloc<double>(n);
+    d.cz = h->alloc<double>(n);
+    d.win = h->alloc<unsigned long long>(n);
+    d.list_cap = static_cast<uint32_t>(n);
+    d.list[0] = h->alloc<uint32_t>(n);
+    d.list[1] = h->alloc<uint32_t>(n);
+    d.mask = h->alloc<uint32_t>(n);
+    d.nnuclei = static_cast<int32_t>(hd.nloc.size());
+    int32_t *nloc = h->alloc<int32_t>(hd.nloc.size());
+    int32_t *ntime = h->alloc<int32_t>(hd.ntime.size());
+    d.nloc = nloc;
+    d.ntime = ntime;
+    float *guv = h->alloc<float>(hd.guv.size());
+    d.guv = guv;
+    std::vector<float> seed(26 * kNumOrientations);
+    for (int o = 0; o < kNumOrientations; o++) seed_critical_lengths(&hd.guv[9 * o], &seed[26 * o]);
+    float *seed_d = h->alloc<float>(seed.size());
+    d.seed_cdl = seed_d;
+    std::vector<double> grow = growth_table(hd.A, hd.B, hd.C, hd.D, hd.dU, 1 << 25);
+    if (grow.size() >= (1u << 25)) throw std::runtime_error("undercooling never saturates within 2^25 steps");
+    double *grow_d = h->alloc<double>(grow.size());
+    d.grow = grow_d;
+    d.ngrow = static_cast<int32_t>(grow.size());
+    h->ctl = h->alloc<StepCtl>(1);
+    h->stats = h->alloc<StepStats>(1);
+    h->check = h->alloc<CheckResult>(1);
+    d.ctl = h->ctl;
+    d.stats = h->stats;
+
+    // Initial state: cell types with the seed bit on the initially active (cell-centred)
+    // octahedra; per-cell undercooling is not stored (it is a function of the steps executed
+    // below the liquidus time).
+    std::vector<uint8_t> ct = hd.ct;
+    for (uint32_t q : hd.active) ct[q] |= kSeedBit;
+    hipStream_t s = h->stream;
+    HIP_OK(hipMemcpyAsync(d.ct, ct.data(), n, hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemcpyAsync(d.gid, hd.gid.data(), n * 4, hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemcpyAsync(d.cts, hd.cts.data(), n * 4, hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemcpyAsync(d.lid, hd.lid.data(), n, hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemsetAsync(d.dl, 0, n * 4, s));
+    HIP_OK(hipMemsetAsync(d.cx, 0, n * 8, s));
+    HIP_OK(hipMemsetAsync(d.cy, 0, n * 8, s));
+    HIP_OK(hipMemsetAsync(d.cz, 0, n * 8, s));
+    HIP_OK(hipMemsetAsync(d.win, 0xFF, n * 8, s));
+    if (!hd.nloc.empty()) {
+        HIP_OK(hipMemcpyAsync(nloc, hd.nloc.data(), hd.nloc.size() * 4, hipMemcpyHostToDevice, s));
+        HIP_OK(hipMemcpyAsync(ntime, hd.ntime.data(), hd.ntime.size() * 4, hipMemcpyHostToDevice, s));
+    }
+    HIP_OK(hipMemcpyAsync(guv, hd.guv.data(), hd.guv.size() * 4, hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemcpyAsync(seed_d, seed.data(), seed.size() * 4, hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemcpyAsync(grow_d, grow.data(), grow.size() * 8, hipMemcpyHostToDevice, s));
+    if (!hd.active.empty()) {
+        HIP_OK(hipMemcpyAsync(d.list[0], hd.active.data(), hd.active.size() * 4, hipMemcpyHostToDevice, s));
+        launch_seed_init(d, d.list[0], static_cast<uint32_t>(hd.active.size()), s);
+        HIP_OK(hipGetLastError());
+    }
+    StepCtl c;
+    std::memset(&c, 0, sizeof c);
+    c.cycle[0] = 1;
+    c.serial[0] = 1;
+    c.count[0] = static_cast<uint32_t>(hd.active.size());
+    HIP_OK(hipMemcpyAsync(h->ctl, &c, sizeof c, hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemsetAsync(h->stats, 0, sizeof(StepStats), s));
+    HIP_OK(hipStreamSynchronize(s));
+}
+
+// Sets the cycle the next step will see. A forward move past the next step records a jump so
+// that the lazily derived undercooling skips the steps that were not executed.
+void set_next_cycle(exaca_state *h, int next) {
+    if (next == h->cycle + 1) return;
+    if (next <= h->cycle) throw StateError("time steps must increase (last " + std::to_string(h->cycle) + ")");
+    if (h->njumps >= exaca::kMaxJumps) throw StateError("too many cycle jumps");
+    const int from = h->cycle, to = next, slot = h->njumps;
+    exaca::StepCtl *c = h->ctl;
+    HIP_OK(hipMemcpyAsync(&c->jump_from[slot], &from, 4, hipMemcpyHostToDevice, h->stream));
+    HIP_OK(hipMemcpyAsync(&c->jump_to[slot], &to, 4, hipMemcpyHostToDevice, h->stream));
+    h->njumps++;
+    HIP_OK(hipMemcpyAsync(&c->njumps, &h->njumps, 4, hipMemcpyHostToDevice, h->stream));
+    HIP_OK(hipMemcpyAsync(&c->cycle[h->parity], &next, 4, hipMemcpyHostToDevice, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));  // host values above are stack temporaries
+    h->cycle = next - 1;
+}
+
+void step_nucleation(exaca_state *h, int layer) {
+    exaca::launch_nucleation(h->dv, h->parity, layer, h->stream);
+}
+void step_capture(exaca_state *h, int layer) {
+    exaca::launch_claim(h->dv, h->parity, layer, h->blocks, h->stream);
+    exaca::launch_apply(h->dv, h->parity, layer, h->blocks, h->stream);
+}
+
+void ensure_graph(exaca_state *h, int layer) {
+    if (h->graph && h->graph_layer == layer) return;
+    if (h->graph) {
+        HIP_OK(hipGraphExecDestroy(h->graph));
+        h->graph = nullptr;
+    }
+    hipGraph_t g;
+    HIP_OK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < kGraphSteps; k++) {
+        const int p = k & 1;
+        exaca::launch_nucleation(h->dv, p, layer, h->stream);
+        exaca::launch_claim(h->dv, p, layer, h->blocks, h->stream);
+        exaca::launch_apply(h->dv, p, layer, h->blocks, h->stream);
+    }
+    HIP_OK(hipStreamEndCapture(h->stream, &g));
+    HIP_OK(hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0));
+    HIP_OK(hipGraphDestroy(g));
+    h->graph_layer = layer;
+}
+
+void run_steps(exaca_state *h, int layer, int n, bool use_graph) {
+    while (n > 0) {
+        if (use_graph && h->parity == 0 && n >= kGraphSteps) {
+            ensure_graph(h, layer);
+            HIP_OK(hipGraphLaunch(h->graph, h->stream));
+            h->cycle += kGraphSteps;
+            n -= kGraphSteps;
+        }
+        else {
+            step_nucleation(h, layer);
+            step_capture(h, layer);
+            h->parity ^= 1;
+            h->cycle += 1;
+            n -= 1;
+        }
+    }
+    HIP_OK(hipGetLastError());
+}
+
+void device_check(exaca_state *h, int *cycle, int layer, uint64_t counts[5], int *xswitch, int verbose) {
+    if (h->pending >= 0) throw StateError("check called between nucleation and capture");
+    if (*cycle != h->cycle) throw StateError("check cycle differs from the last executed step");
+    exaca::CheckResult r;
+    std::memset(&r, 0, sizeof r);
+    r.next_cts = ~0ull;
+    HIP_OK(hipMemcpyAsync(h->check, &r, sizeof r, hipMemcpyHostToDevice, h->stream));
+    exaca::launch_check(h->dv, *cycle, layer, h->check, h->stream);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(&r, h->check, sizeof r, hipMemcpyDeviceToHost, h->stream));
+    exaca::StepStats st;
+    int32_t err = 0;
+    HIP_OK(hipMemcpyAsync(&st, h->stats, sizeof st, hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipMemcpyAsync(&err, &h->ctl->error, 4, hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    if (err) throw DeviceError("device error flag " + std::to_string(err) + " (1: frontier overflow, 2: grain 0)");
+    if (counts)
+        for (int k = 0; k < 5; k++) counts[k] = r.counts[k];
+    if (verbose)
+        std::cout << "cycle = " << *cycle << " Superheated liquid cells = " << r.counts[0]
+                  << " Undercooled liquid cells = " << r.counts[1] << " Number of nucleation events this layer "
+                  << st.nucleated << " Remaining liquid cells in future layers of this simulation = " << r.counts[4]
+                  << std::endl;
+    if (r.counts[0] + r.counts[1] == 0) *xswitch = 1;
+    // Spot-melt problems skip ahead when the next liquidus crossing is more than 5000 steps away
+    // (src/CAupdate.cpp:787-817; unsigned long arithmetic as there).
+    if (*xswitch == 0 && h->deck.sim_type == "S" && r.counts[1] == 0) {
+        const unsigned long next = static_cast<unsigned long>(r.next_cts);
+        if ((next - static_cast<unsigned long>(static_cast<long>(*cycle))) > 5000) {
+            set_next_cycle(h, static_cast<int>(next));
+            *cycle = h->cycle;
+            if (verbose) std::cout << "Jumping to cycle " << *cycle + 1 << std::endl;
+        }
+    }
+}
+
+void run_layer(exaca_state *h, int layer, int max_cycles, int verbose, int *out_cycle, double *gpu_ms) {
+    if (h->pending >= 0) throw StateError("run_layer called between nucleation and capture");
+    const bool graphs = std::getenv("EXACA_NO_GRAPHS") == nullptr;
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    HIP_OK(hipEventRecord(e0, h->stream));
+    int xswitch = 0;
+    do {
+        int target = (h->cycle / 1000 + 1) * 1000;
+        if (max_cycles > 0 && target > max_cycles) target = max_cycles;
+        run_steps(h, layer, target - h->cycle, graphs);
+        if (h->cycle % 1000 == 0) {
+            int c = h->cycle;
+            device_check(h, &c, layer, nullptr, &xswitch, verbose);
+        }
+        if (max_cycles > 0 && h->cycle >= max_cycles) break;
+    } while (xswitch == 0);
+    HIP_OK(hipEventRecord(e1, h->stream));
+    HIP_OK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    int32_t err = 0;
+    HIP_OK(hipMemcpy(&err, &h->ctl->error, 4, hipMemcpyDeviceToHost));
+    if (err) throw DeviceError("device error flag " + std::to_string(err) + " (1: frontier overflow, 2: grain 0)");
+    if (out_cycle) *out_cycle = h->cycle;
+    if (gpu_ms) *gpu_ms = ms;
+}
+
+exaca_state *create_state(const std::string &input, const std::string &dd, bool verbose) {
+    auto t0 = std::chrono::steady_clock::now();
+    auto h = std::make_unique<exaca_state>();
+    h->deck = exaca::read_deck(input, dd);
+    h->host = exaca::initialize(h->deck, verbose);
+    build_device_state(h.get());
+    h->init_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return h.release();
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *exaca_last_error(void) { return g_last_error.c_str(); }
+
+int exaca_create(const char *input_file, const char *data_dir, int verbose, exaca_state **out) {
+    return guarded([&] {
+        if (!input_file || !out) throw std::invalid_argument("null argument");
+        *out = nullptr;
+        *out = create_state(input_file, data_dir ? data_dir : exaca::data_dir(), verbose != 0);
+    });
+}
+
+int exaca_nucleation(exaca_state *h, int cycle, int layer) {
+    return guarded([&] {
+        if (!h) throw std::invalid_argument("null handle");
+        if (h->pending >= 0) throw StateError("nucleation called twice without capture");
+        set_next_cycle(h, cycle);
+        step_nucleation(h, layer);
+        HIP_OK(hipGetLastError());
+        h->pending = cycle;
+    });
+}
+
+int exaca_cell_capture(exaca_state *h, int cycle, int layer) {
+    return guarded([&] {
+        if (!h) throw std::invalid_argument("null handle");
+        if (h->pending != cycle) throw StateError("capture requires nucleation of the same cycle first");
+        step_capture(h, layer);
+        HIP_OK(hipGetLastError());
+        h->parity ^= 1;
+        h->cycle = cycle;
+        h->pending = -1;
+    });
+}
+
+int exaca_check(exaca_state *h, int *cycle, int layer, uint64_t counts[5], int *xswitch, int verbose) {
+    return guarded([&] {
+        if (!h || !cycle || !xswitch) throw std::invalid_argument("null argument");
+        device_check(h, cycle, layer, counts, xswitch, verbose);
+    });
+}
+
+int exaca_run_layer(exaca_state *h, int layer, int max_cycles, int verbose, int *cycle, double *gpu_ms) {
+    return guarded([&] {
+        if (!h) throw std::invalid_argument("null handle");
+        run_layer(h, layer, max_cycles, verbose, cycle, gpu_ms);
+    });
+}
+
+int exaca_download(exaca_state *h, int field, void *dst, size_t dst_bytes) {
+    return guarded([&] {
+        if (!h || !dst) throw std::invalid_argument("null argument");
+        const size_t n = static_cast<size_t>(h->dv.ncell);
+        const exaca::DomainView &d = h->dv;
+        auto need = [&](size_t b) {
+            if (dst_bytes < b) throw std::invalid_argument("destination too small");
+        };
+        HIP_OK(hipStreamSynchronize(h->stream));
+        switch (field) {
+        case EXACA_FIELD_CELLTYPE:
+        case EXACA_FIELD_CELLTYPE_RAW: {
+            std::vector<uint8_t> t(n);
+            HIP_OK(hipMemcpy(t.data(), d.ct, n, hipMemcpyDeviceToHost));
+            if (field == EXACA_FIELD_CELLTYPE_RAW) {
+                need(n);
+                std::memcpy(dst, t.data(), n);
+            }
+            else {
+                need(4 * n);
+                int32_t *o = static_cast<int32_t *>(dst);
+                for (size_t q = 0; q < n; q++) o[q] = t[q] & exaca::kTypeMask;
+            }
+            break;
+        }
+        case EXACA_FIELD_GRAINID:
+            need(4 * n);
+            HIP_OK(hipMemcpy(dst, d.gid, 4 * n, hipMemcpyDeviceToHost));
+            break;
+        case EXACA_FIELD_CRITTIMESTEP:
+            need(4 * n);
+            HIP_OK(hipMemcpy(dst, d.cts, 4 * n, hipMemcpyDeviceToHost));
+            break;
+        case EXACA_FIELD_DIAGONAL:
+            need(4 * n);
+            HIP_OK(hipMemcpy(dst, d.dl, 4 * n, hipMemcpyDeviceToHost));
+            break;
+        case EXACA_FIELD_LAYERID: {
+            need(4 * n);
+            std::vector<int8_t> t(n);
+            HIP_OK(hipMemcpy(t.data(), d.lid, n, hipMemcpyDeviceToHost));
+            int32_t *o = static_cast<int32_t *>(dst);
+            for (size_t q = 0; q < n; q++) o[q] = t[q];
+            break;
+        }
+        case EXACA_FIELD_DOCENTER: {
+            need(12 * n);
+            std::vector<double> t(n);
+            float *o = static_cast<float *>(dst);
+            double *src[3] = {d.cx, d.cy, d.cz};
+            for (int a = 0; a < 3; a++) {
+                HIP_OK(hipMemcpy(t.data(), src[a], 8 * n, hipMemcpyDeviceToHost));
+                for (size_t q = 0; q < n; q++) o[3 * q + a] = static_cast<float>(t[q]);
+            }
+            break;
+        }
+        default:
+            throw std::invalid_argument("unknown field " + std::to_string(field));
+        }
+    });
+}
+
+int exaca_query(const exaca_state *h, int64_t out[16]) {
+    return guarded([&] {
+        if (!h || !out) throw std::invalid_argument("null argument");
+        exaca::StepStats st;
+        exaca::StepCtl c;
+        HIP_OK(hipStreamSynchronize(h->stream));
+        HIP_OK(hipMemcpy(&st, h->stats, sizeof st, hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(&c, h->ctl, sizeof c, hipMemcpyDeviceToHost));
+        const exaca::HostDomain &hd = h->host;
+        out[0] = hd.X;
+        out[1] = hd.Y;
+        out[2] = hd.nz;
+        out[3] = hd.nzA;
+        out[4] = hd.nx;
+        out[5] = hd.ny;
+        out[6] = static_cast<int64_t>(hd.nloc.size());
+        out[7] = h->cycle;
+        out[8] = static_cast<int64_t>(st.nucleated);
+        out[9] = static_cast<int64_t>(st.frontier);
+        out[10] = static_cast<int64_t>(st.captures);
+        out[11] = static_cast<int64_t>(st.solidified);
+        out[12] = static_cast<int64_t>(st.waiting);
+        out[13] = c.count[h->parity];
+        out[14] = h->dv.list_cap;
+        out[15] = static_cast<int64_t>(h->bytes);
+    });
+}
+
+int exaca_destroy(exaca_state *h) {
+    return guarded([&] { delete h; });
+}
+
+int exaca_geometry_probe(exaca_state *h, const double *in, double *out, int n) {
+    return guarded([&] {
+        if (!h || !in || !out || n <= 0) throw std::invalid_argument("bad argument");
+        for (int r = 0; r < n; r++) {
+            const double o = in[7 * r];
+            if (!(o >= 0 && o < exaca::kNumOrientations)) throw std::invalid_argument("orientation out of range");
+        }
+        double *din = nullptr, *dout = nullptr;
+        HIP_OK(hipMalloc(&din, sizeof(double) * 7 * n));
+        HIP_OK(hipMalloc(&dout, sizeof(double) * 30 * n));
+        HIP_OK(hipMemcpy(din, in, sizeof(double) * 7 * n, hipMemcpyHostToDevice));
+        exaca::launch_geometry_probe(h->dv.guv, din, dout, n, h->stream);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipStreamSynchronize(h->stream));
+        HIP_OK(hipMemcpy(out, dout, sizeof(double) * 30 * n, hipMemcpyDeviceToHost));
+        (void)hipFree(din);
+        (void)hipFree(dout);
+    });
+}
+
+int exaca_host_state(const char *input_file, const char *data_dir, int field, void *dst, size_t dst_bytes,
+                     int64_t dims[8]) {
+    return guarded([&] {
+        if (!input_file || !dims) throw std::invalid_argument("null argument");
+        const exaca::Deck deck = exaca::read_deck(input_file, data_dir ? data_dir : exaca::data_dir());
+        const exaca::HostDomain hd = exaca::initialize(deck, false);
+        dims[0] = hd.X;
+        dims[1] = hd.Y;
+        dims[2] = hd.nz;
+        dims[3] = hd.nzA;
+        dims[4] = hd.nx;
+        dims[5] = hd.ny;
+        dims[6] = static_cast<int64_t>(hd.nloc.size());
+        dims[7] = static_cast<int64_t>(hd.active.size());
+        if (!dst) return;
+        const size_t n = static_cast<size_t>(hd.ncell);
+        auto put = [&](const int32_t *src, size_t count) {
+            if (dst_bytes < 4 * count) throw std::invalid_argument("destination too small");
+            std::memcpy(dst, src, 4 * count);
+        };
+        std::vector<int32_t> t;
+        switch (field) {
+        case EXACA_FIELD_CELLTYPE:
+            t.assign(hd.ct.begin(), hd.ct.end());
+            put(t.data(), n);
+            break;
+        case EXACA_FIELD_GRAINID: put(hd.gid.data(), n); break;
+        case EXACA_FIELD_CRITTIMESTEP: put(hd.cts.data(), n); break;
+        case EXACA_FIELD_LAYERID:
+            t.assign(hd.lid.begin(), hd.lid.end());
+            put(t.data(), n);
+            break;
+        case EXACA_FIELD_NUCLEI_LOC: put(hd.nloc.data(), hd.nloc.size()); break;
+        case EXACA_FIELD_NUCLEI_TIME: put(hd.ntime.data(), hd.ntime.size()); break;
+        default: throw std::invalid_argument("field not available from the host state");
+        }
+    });
+}
+
+int exaca_run_program(int id, int np, const char *input_file) {
+    return guarded([&] {
+        if (!input_file) throw std::invalid_argument("Error: Must provide path to input file on the command line.");
+        if (np != 1) throw std::invalid_argument("one GPU holds the whole domain: np must be 1");
+        (void)id;
+        auto t0 = std::chrono::steady_clock::now();
+        std::unique_ptr<exaca_state> h(create_state(input_file, exaca::data_dir(), true));
+        std::cout << "Data initialized: Time spent: " << h->init_seconds << " s" << std::endl;
+        int cycle = 0;
+        double ms = 0;
+        auto t1 = std::chrono::steady_clock::now();
+        run_layer(h.get(), 0, 0, 1, &cycle, &ms);
+        auto t2 = std::chrono::steady_clock::now();
+        exaca::RunTimes times;
+        times.init = std::chrono::duration<double>(t1 - t0).count();
+        times.run = std::chrono::duration<double>(t2 - t1).count();
+        times.capture = ms / 1000.0;
+        std::cout << "Time for final layer was " << times.run << " s" << std::endl;
+        const exaca::Deck &d = h->deck;
+        if (d.print_misorientation || d.print_full) {
+            std::cout << "Collecting data on rank 0 and printing to files" << std::endl;
+            std::vector<int32_t> gid(static_cast<size_t>(h->dv.ncell));
+            HIP_OK(hipMemcpy(gid.data(), h->dv.gid, gid.size() * 4, hipMemcpyDeviceToHost));
+            exaca::write_vtk_files(d, h->host, gid);
+        }
+        else {
+            std::cout << "No output files to be printed, exiting program" << std::endl;
+        }
+        times.out = std::chrono::duration<double>(std::chrono::steady_clock::now() - t2).count();
+        exaca::write_log(d, h->host, cycle, times);
+        std::cout << "===================================================================================" << std::endl;
+        std::cout << "Having run with = 1 processors" << std::endl;
+        std::cout << "Output written at cycle = " << cycle << std::endl;
+        std::cout << "Total time = " << times.init + times.run + times.out << std::endl;
+        std::cout << "Time spent initializing data = " << times.init << " s" << std::endl;
+        std::cout << "Time spent performing CA calculations = " << times.run << " s" << std::endl;
+        std::cout << "Time spent collecting and printing output data = " << times.out << " s\n" << std::endl;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,402 @@
+// init.cpp -- host construction of the initial CA state for one rank (np = 1). Each function
+// names the reference routine whose loop order and arithmetic it keeps; the random streams are
+// libstdc++'s, drawn in the reference's order, so the state is identical cell for cell.
+#include "init.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <thread>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <iostream>
+#include <random>
+#include <sstream>
+#include <stdexcept>
+
+#include "ca_common.hpp"
+
+namespace exaca {
+namespace {
+
+inline long long cell(const HostDomain &h, int k, int i, int j) {
+    return (static_cast<long long>(k) * h.X + i) * h.Y + j;
+}
+
+// FindItBounds-equivalent neighbour filter: a neighbour exists when it stays inside the rank's
+// X-Y slab (src/CAfunctions.cpp:169-213).
+inline bool in_slab(const HostDomain &h, int i, int j, int n) {
+    int x = i + kNbX[n], y = j + kNbY[n];
+    return x >= 0 && x < h.X && y >= 0 && y < h.Y;
+}
+
+// TempInit_DirSolidification (src/CAinitialize.cpp:1025-1055).
+void temperature_directional(const Deck &d, HostDomain &h) {
+    h.zlow = 0;
+    h.zhigh = h.nz - 1;
+    h.nzA = h.nz;
+    h.dU = static_cast<float>(d.R * d.deltat);
+    for (int k = 0; k < h.nz; k++) {
+        int crit = static_cast<int>(((k - 1) * d.G * d.deltax) / (d.R * d.deltat));
+        for (int i = 0; i < h.X; i++)
+            for (int j = 0; j < h.Y; j++) {
+                long long p = cell(h, k, i, j);
+                h.lid[p] = 0;
+                h.cts[p] = crit;
+                int gx = i + h.xoff, gy = j + h.yoff;
+                h.melted[p] = (gx > -1 && gx < h.nx && gy > -1 && gy < h.ny && k > 0 && k < h.nz - 1);
+            }
+    }
+}
+
+// TempInit_SpotMelt (src/CAinitialize.cpp:1058-1124). Cells outside every spot keep
+// CritTimeStep 0: the reference leaves them unset in a fresh (zero) allocation.
+void temperature_spots(const Deck &d, HostDomain &h) {
+    h.zlow = 0;
+    h.zhigh = d.spot_radius + 1;
+    h.nzA = d.spot_radius + 2;
+    h.dU = static_cast<float>(d.R * d.deltat);
+    std::fill(h.lid.begin(), h.lid.end(), static_cast<int8_t>(-1));
+    const int nspots = d.nspots_x * d.nspots_y;
+    const float iso = static_cast<float>((d.R / d.G) * d.deltat / d.deltax);
+    const int between = static_cast<int>(d.spot_radius / iso + (d.freezing_range / d.R) / d.deltat);
+    for (int layer = 0; layer < d.num_layers; layer++)
+        for (int n = 0; n < nspots; n++) {
+            const int xs = 2 + d.spot_radius + (n % d.nspots_x) * d.spot_offset;
+            const int ys = 2 + d.spot_radius + (n / d.nspots_x) * d.spot_offset;
+            const int zs = d.spot_radius + 2 + d.layer_height * layer;
+            for (int k = 0; k < h.nzA; k++) {
+                const float dz = static_cast<float>(zs - (k + d.layer_height * layer));
+                for (int i = 0; i < h.X; i++) {
+                    const float dx = static_cast<float>(xs - (i + h.xoff));
+                    for (int j = 0; j < h.Y; j++) {
+                        const float dy = static_cast<float>(ys - (j + h.yoff));
+                        const float dist = std::sqrt(dx * dx + dy * dy + dz * dz);
+                        if (dist <= d.spot_radius) {
+                            long long p = cell(h, k + layer * d.layer_height, i, j);
+                            h.cts[p] = 1 + static_cast<int>((static_cast<float>(d.spot_radius) - dist) / iso) +
+                                       between * n;
+                            h.lid[p] = static_cast<int8_t>(layer);
+                            h.melted[p] = 1;
+                        }
+                    }
+                }
+            }
+        }
+}
+
+// SubstrateInit_ConstrainedGrowth (src/CAinitialize.cpp:1460-1529), rank 0 of 1.
+void substrate_directional(const Deck &d, HostDomain &h) {
+    std::mt19937_64 gen(0);
+    std::uniform_real_distribution<double> dis(0.0, 1.0);
+    for (int k = 0; k < h.nz; k++)
+        for (int i = 0; i < h.X; i++)
+            for (int j = 0; j < h.Y; j++) {
+                const int gx = i + h.xoff, gy = j + h.yoff;
+                long long p = cell(h, k, i, j);
+                if (gx == -1 || gx == h.nx || gy == -1 || gy == h.ny || k == 0 || k == h.nz - 1) {
+                    h.ct[p] = kWall;
+                    h.gid[p] = 0;
+                }
+                else if (k == 1 && i > 0 && i < h.X - 1 && j > 0 && j < h.Y - 1) {
+                    h.ct[p] = dis(gen) < d.fract_active ? kActive : kLiquid;
+                }
+                else {
+                    h.ct[p] = kLiquid;
+                }
+            }
+    int next = 1;
+    for (long long p = 0; p < h.ncell; p++)
+        if (h.ct[p] == kActive) {
+            h.gid[p] = next++;
+            h.substrate_active++;
+        }
+}
+
+// SubstrateInit_FromFile (src/CAinitialize.cpp:1532-1592).
+void substrate_file(const Deck &d, HostDomain &h) {
+    std::ifstream in(d.substrate_file);
+    std::string s;
+    int dims[3];
+    for (int q = 0; q < 3; q++) {
+        std::getline(in, s);
+        dims[q] = std::stoi(s.substr(s.find('=') + 1), nullptr, 10);
+    }
+    const int nzS = dims[0], nyS = dims[1], nxS = dims[2];
+    if (nzS < h.nz)
+        throw std::runtime_error("Error: only " + std::to_string(nzS) + " layers of substrate data are present in the file " +
+                                 d.substrate_file + " ; at least " + std::to_string(h.nz) +
+                                 " layers of substrate data are required to simulate the specified solidification problem");
+    for (int k = 0; k < nzS && k != h.nz; k++)
+        for (int j = 0; j < nyS; j++)
+            for (int i = 0; i < nxS; i++) {
+                std::getline(in, s);
+                if (i >= h.xoff && i < h.xoff + h.X && j >= h.yoff && j < h.yoff + h.Y) {
+                    long long p = cell(h, k, i - h.xoff, j - h.yoff);
+                    h.gid[p] = (h.cts[p] == 0) ? std::stoi(s, nullptr, 10) : 0;
+                }
+            }
+}
+
+// SubstrateInit_FromGrainSpacing (src/CAinitialize.cpp:1597-1732): baseplate seeds drawn from
+// mt19937_64(1) over the global grid, then a Voronoi assignment of the substrate cells in the
+// first layer's active window (nearest seed by float distance; the lowest seed index wins ties,
+// as with the reference's strict < scan), then one new grain per powder-layer cell above it.
+void substrate_spacing(const Deck &d, HostDomain &h) {
+    std::mt19937_64 gen(1);
+    std::uniform_real_distribution<double> dis(0.0, 1.0);
+    const float sp = d.substrate_spacing;
+    const double prob = (d.deltax * d.deltax * d.deltax) / (sp * sp * sp * std::pow(10, -18));
+    std::vector<int> bx, by, bz;
+    for (int k = 1; k <= h.nzA; k++)
+        for (int i = 1; i < h.nx - 1; i++)
+            for (int j = 1; j < h.ny - 1; j++)
+                if (dis(gen) < prob) {
+                    bx.push_back(i);
+                    by.push_back(j);
+                    bz.push_back(k);
+                }
+    const int nb = static_cast<int>(bx.size());
+    h.baseplate_grains = nb;
+    const float far = static_cast<float>(static_cast<long long>(h.X) * h.Y * h.nzA);
+    // Each cell's nearest seed is independent of every other cell: planes are shared out
+    // across host threads.
+    auto plane = [&](int k) {
+        for (int i = 0; i < h.X; i++)
+            for (int j = 0; j < h.Y; j++) {
+                const int gx = i + h.xoff, gy = j + h.yoff;
+                long long p = cell(h, k, i, j);
+                if (h.cts[p] != 0) {
+                    h.gid[p] = 0;
+                    continue;
+                }
+                float best = far;
+                int closest = -1;
+                for (int n = 0; n < nb; n++) {
+                    const float ex = static_cast<float>(std::abs(bx[n] - gx));
+                    const float ey = static_cast<float>(std::abs(by[n] - gy));
+                    const float ez = static_cast<float>(std::abs(bz[n] - k));
+                    const float dist = sqrtf(ex * ex + ey * ey + ez * ez);
+                    if (dist < best) {
+                        closest = n;
+                        best = dist;
+                    }
+                }
+                h.gid[p] = closest + 1;
+            }
+    };
+    std::atomic<int> next_plane{1};
+    const unsigned nthreads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < nthreads; t++)
+        pool.emplace_back([&] {
+            for (int k = next_plane++; k < h.nzA; k = next_plane++) plane(k);
+        });
+    for (auto &th : pool) th.join();
+    for (int k = h.nzA; k < h.nz; k++)
+        for (int i = 1; i < h.X - 1; i++)
+            for (int j = 1; j < h.Y - 1; j++) {
+                long long p = cell(h, k, i, j);
+                if (h.cts[p] != 0) h.gid[p] = 0;
+            }
+    int next = nb + 1;
+    for (long long p = static_cast<long long>(h.X) * h.Y * h.nzA; p < h.ncell; p++)
+        if (h.cts[p] == 0) h.gid[p] = next++;
+}
+
+// ActiveCellWallInit (src/CAinitialize.cpp:1736-1830) with ExtraWalls = false: walls on the
+// ghost ring and the top/bottom planes, solid where no melting data exists, liquid elsewhere,
+// then solid cells touching a melted non-wall neighbour become active.
+void walls_and_interface(HostDomain &h) {
+    for (int k = 0; k < h.nz; k++)
+        for (int i = 0; i < h.X; i++)
+            for (int j = 0; j < h.Y; j++) {
+                const int gx = i + h.xoff, gy = j + h.yoff;
+                if (gx == -1 || gx == h.nx || gy == -1 || gy == h.ny || k == 0 || k == h.nz - 1) {
+                    long long p = cell(h, k, i, j);
+                    h.ct[p] = kWall;
+                    h.gid[p] = 0;
+                }
+            }
+    for (int k = 1; k < h.nz - 1; k++)
+        for (int i = 0; i < h.X; i++)
+            for (int j = 0; j < h.Y; j++) {
+                long long p = cell(h, k, i, j);
+                h.ct[p] = (h.cts[p] == 0) ? kSolid : kLiquid;
+            }
+    // A cell's activation depends only on its neighbours' CritTimeStep and on whether they are
+    // walls, neither of which this loop changes, so the scan order does not matter.
+    for (int k = 1; k < h.nz - 1; k++)
+        for (int i = 0; i < h.X; i++)
+            for (int j = 0; j < h.Y; j++) {
+                long long p = cell(h, k, i, j);
+                if (h.cts[p] != 0) continue;
+                for (int n = 0; n < 26; n++) {
+                    if (!in_slab(h, i, j, n)) continue;
+                    long long q = cell(h, k + kNbZ[n], i + kNbX[n], j + kNbY[n]);
+                    if (h.cts[q] > 0 && h.ct[q] != kWall) {
+                        h.ct[p] = kActive;
+                        h.substrate_active++;
+                        break;
+                    }
+                }
+            }
+}
+
+// GrainInit (src/CAinitialize.cpp:1834-2206), first layer of rank 0 of 1: potential nuclei drawn
+// from mt19937_64(0) over cells with melting data in (z, y, x) order, numbered -1, -2, ... in
+// index order; then liquid cells with no non-solid, non-wall neighbour turn solid, in index order.
+void grains(const Deck &d, HostDomain &h) {
+    std::mt19937_64 gen(0);
+    std::uniform_real_distribution<double> dis(0.0, 1.0);
+    const double bulk = d.nmax * d.deltax * d.deltax * d.deltax;
+    for (int k = 1; k < h.nz - 1; k++)
+        for (int j = 1; j < h.Y - 1; j++)
+            for (int i = 1; i < h.X - 1; i++) {
+                long long p = cell(h, k, i, j);
+                if (h.cts[p] != 0 && dis(gen) < bulk) h.ct[p] = kTemporaryInit;
+            }
+    int ncounter = -1;
+    const long long lo = static_cast<long long>(h.X) * h.Y, hi = static_cast<long long>(h.X) * h.Y * (h.nz - 1);
+    for (long long p = lo; p < hi; p++) {
+        if (h.ct[p] == kActive) {
+            if (p / (static_cast<long long>(h.X) * h.Y) <= h.zhigh) h.active.push_back(static_cast<uint32_t>(p));
+        }
+        else if (h.ct[p] == kTemporaryInit) {
+            h.gid[p] = ncounter--;
+        }
+    }
+    for (int k = 1; k < h.nz - 1; k++)
+        for (int i = 1; i < h.X - 1; i++)
+            for (int j = 1; j < h.Y - 1; j++) {
+                long long p = cell(h, k, i, j);
+                if (h.ct[p] != kLiquid) continue;
+                bool bordered = false;
+                for (int n = 0; n < 26 && !bordered; n++) {
+                    uint8_t t = h.ct[cell(h, k + kNbZ[n], i + kNbX[n], j + kNbY[n])];
+                    bordered = (t != kSolid && t != kWall);
+                }
+                if (!bordered) h.ct[p] = kSolid;
+            }
+}
+
+// NucleiInit (src/CAinitialize.cpp:2213-2248, 2764-2767), rank 0: one normal draw per site in
+// index order from a default-seeded std::default_random_engine.
+void nuclei(const Deck &d, HostDomain &h) {
+    std::default_random_engine generator;
+    std::normal_distribution<double> distribution(d.dTN, d.dTsigma);
+    for (long long p = 0; p < h.ncell; p++) {
+        if (h.ct[p] != kTemporaryInit) continue;
+        const double und = distribution(generator);
+        const int t = static_cast<int>(h.cts[p] + std::round(und / h.dU));
+        h.nloc.push_back(static_cast<int32_t>(p));
+        h.ntime.push_back(std::max(h.cts[p], t));
+        h.ct[p] = kLiquid;
+    }
+}
+
+}  // namespace
+
+std::vector<float> read_orientations(const std::string &path) {
+    std::vector<float> guv(9 * kNumOrientations, 0.0f);
+    std::ifstream in(path);
+    if (!in.is_open()) throw std::runtime_error("Could not locate/open Substrate file");
+    std::string line;
+    std::getline(in, line);
+    for (int i = 0; i < kNumOrientations; i++) {
+        if (!std::getline(in, line)) break;
+        std::istringstream ss(line);
+        std::string tok;
+        int slot = 0;
+        while (slot < 9 && std::getline(ss, tok, ',')) guv[9 * i + slot++] = static_cast<float>(std::atof(tok.c_str()));
+    }
+    return guv;
+}
+
+void seed_critical_lengths(const float *g, float out[26]) {
+    for (int n = 0; n < 26; n++) {
+        const double x0 = kNbX[n], y0 = kNbY[n], z0 = kNbZ[n];
+        const double mag0 = std::pow(std::pow(x0, 2.0) + std::pow(y0, 2.0) + std::pow(z0, 2.0), 0.5);
+        double diag[3][3];
+        for (int a = 0; a < 3; a++) {
+            const double ang = (g[3 * a] * x0 + g[3 * a + 1] * y0 + g[3 * a + 2] * z0) / mag0;
+            for (int c = 0; c < 3; c++) diag[a][c] = (ang < 0) ? g[3 * a + c] : -g[3 * a + c];
+        }
+        const double u1[3] = {diag[1][0] - diag[0][0], diag[1][1] - diag[0][1], diag[1][2] - diag[0][2]};
+        const double u2[3] = {diag[2][0] - diag[0][0], diag[2][1] - diag[0][1], diag[2][2] - diag[0][2]};
+        const double uu[3] = {u1[1] * u2[2] - u1[2] * u2[1], u1[2] * u2[0] - u1[0] * u2[2],
+                              u1[0] * u2[1] - u1[1] * u2[0]};
+        const double nd = std::sqrt(uu[0] * uu[0] + uu[1] * uu[1] + uu[2] * uu[2]);
+        const double nrm[3] = {uu[0] / nd, uu[1] / nd, uu[2] / nd};
+        const double t = (nrm[0] * x0 + nrm[1] * y0 + nrm[2] * z0) /
+                         (nrm[0] * diag[0][0] + nrm[1] * diag[0][1] + nrm[2] * diag[0][2]);
+        out[n] = static_cast<float>(std::pow(
+            std::pow(t * diag[0][0], 2.0) + std::pow(t * diag[0][1], 2.0) + std::pow(t * diag[0][2], 2.0), 0.5));
+    }
+}
+
+std::vector<double> growth_table(double A, double B, double C, double D, float dU, int max_entries) {
+    // U_k = U_{k-1} + dU in float from U_0 = 0 (src/CAupdate.cpp:236-237; TempInit sets U = 0);
+    // increment = min(0.045, max(0, A U^3 + B U^2 + C U + D)) with U clamped at 210 (:257-261).
+    std::vector<double> t;
+    float u = 0.0f;
+    for (int k = 0; k < max_entries; k++) {
+        double lu = u;
+        lu = (lu < 210.0) ? lu : 210.0;
+        double v = A * std::pow(lu, 3.0) + B * std::pow(lu, 2.0) + C * lu + D;
+        v = (0.0 < v) ? v : 0.0;
+        t.push_back((v < 0.045) ? v : 0.045);
+        if (lu >= 210.0) break;
+        float next = u + dU;
+        if (next == u) break;
+        u = next;
+    }
+    return t;
+}
+
+HostDomain initialize(const Deck &d, bool verbose) {
+    HostDomain h;
+    h.nx = d.nx;
+    h.ny = d.ny;
+    h.nz = d.nz;
+    h.X = d.nx + 2;
+    h.Y = d.ny + 2;
+    h.xoff = -1;
+    h.yoff = -1;
+    h.ncell = static_cast<long long>(h.X) * h.Y * h.nz;
+    if (h.ncell >= (1LL << 32) - 1) throw std::runtime_error("domain exceeds 2^32 cells");
+    h.ct.assign(h.ncell, 0);
+    h.gid.assign(h.ncell, 0);
+    h.cts.assign(h.ncell, 0);
+    h.lid.assign(h.ncell, 0);
+    h.melted.assign(h.ncell, 0);
+    if (d.sim_type == "C")
+        temperature_directional(d, h);
+    else
+        temperature_spots(d, h);
+    h.guv = read_orientations(d.orientation_file);
+    if (d.sim_type == "C") {
+        substrate_directional(d, h);
+    }
+    else {
+        if (d.use_substrate_file)
+            substrate_file(d, h);
+        else
+            substrate_spacing(d, h);
+        walls_and_interface(h);
+    }
+    grains(d, h);
+    nuclei(d, h);
+    h.A = d.A * d.deltat / d.deltax;
+    h.B = d.B * d.deltat / d.deltax;
+    h.C = d.C * d.deltat / d.deltax;
+    h.D = d.D;
+    if (verbose) {
+        std::cout << "Number of substrate active cells: " << h.substrate_active << std::endl;
+        std::cout << "Number of potential nucleated grains: " << h.nloc.size() << std::endl;
+    }
+    return h;
+}
+
+}  // namespace exaca

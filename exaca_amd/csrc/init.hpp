@@ -1,0 +1,45 @@
+// init.hpp -- one-rank initial state of a CA domain, built on the host the way the reference's
+// serial init loops build it (src/CAinitialize.cpp:1025-2248).
+#pragma once
+#include <cstdint>
+#include <vector>
+
+#include "deck.hpp"
+
+namespace exaca {
+
+struct HostDomain {
+    int nx = 0, ny = 0, nz = 0;   // global sizes including the wall planes
+    int X = 0, Y = 0;             // MyXSlices, MyYSlices
+    int xoff = -1, yoff = -1;     // MyXOffset, MyYOffset
+    int nzA = 0, zlow = 0, zhigh = 0;
+    long long ncell = 0;          // X*Y*nz
+    std::vector<uint8_t> ct;      // reference cell types (no seed bit)
+    std::vector<int32_t> gid, cts;
+    std::vector<int8_t> lid;
+    std::vector<uint8_t> melted;
+    float dU = 0;                 // UndercoolingChange shared by every cell that ever accumulates
+    std::vector<int32_t> nloc, ntime;  // nucleation sites and steps (NucleiLocation/NucleationTimes)
+    std::vector<float> guv;       // GrainUnitVector, 10000 x 9
+    std::vector<uint32_t> active; // initially active cells, ascending index
+    // normalised interfacial response (A, B, C scaled by deltat/deltax; D unscaled, runCA.cpp:198-200)
+    double A = 0, B = 0, C = 0, D = 0;
+    int substrate_active = 0, baseplate_grains = 0;
+};
+
+// Runs TempInit_{DirSolidification,SpotMelt}, OrientationInit, the substrate init for the deck's
+// problem type, ActiveCellWallInit (type S), GrainInit and NucleiInit for rank 0 of 1.
+HostDomain initialize(const Deck &deck, bool verbose);
+
+// The 26 critical diagonal lengths of a cell-centred octahedron of one orientation
+// (src/CAupdate.cpp:112-194), evaluated with the host libm exactly as the reference does.
+void seed_critical_lengths(const float *guv9, float out26[26]);
+
+// Growth increment per step after k executed steps below the liquidus, k = 0 .. n-1; the last
+// entry is the saturated value (undercooling clamped at 210 K or no longer increasing).
+std::vector<double> growth_table(double A, double B, double C, double D, float dU, int max_entries);
+
+// Orientation file reader (src/CAinitialize.cpp:1408-1442).
+std::vector<float> read_orientations(const std::string &path);
+
+}  // namespace exaca

@@ -1,0 +1,104 @@
+/*
+ * exaca_ca.h -- C ABI of the MI355X-native ExaCA cellular-automaton time step.
+ *
+ * Plain C types only: an opaque handle, integers, pointers and sizes. Every entry point returns
+ * 0 on success and a nonzero status on failure (the message is available from
+ * exaca_last_error()); no exception crosses this boundary. One host thread per handle; each
+ * handle owns one HIP stream on the current device.
+ *
+ * Each declaration names the reference interface it replaces (paths relative to the ExaCA
+ * repository root).
+ */
+#ifndef EXACA_CA_H
+#define EXACA_CA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct exaca_state exaca_state;
+
+enum exaca_status {
+    EXACA_OK = 0,
+    EXACA_ERR_INPUT = 1,   /* bad deck / file / argument (the reference throws std::runtime_error) */
+    EXACA_ERR_DEVICE = 2,  /* HIP runtime failure or device-side error flag */
+    EXACA_ERR_STATE = 3,   /* call out of order (e.g. capture without nucleation for that cycle) */
+};
+
+/* Field selectors for exaca_download. */
+enum exaca_field {
+    EXACA_FIELD_CELLTYPE = 0,     /* int32, reference TypeNames values (src/CAtypes.hpp:11) */
+    EXACA_FIELD_GRAINID = 1,      /* int32 */
+    EXACA_FIELD_DIAGONAL = 2,     /* float, DiagonalLength */
+    EXACA_FIELD_DOCENTER = 3,     /* float[3] per cell, DOCenter (x, y, z interleaved) */
+    EXACA_FIELD_CRITTIMESTEP = 4, /* int32 */
+    EXACA_FIELD_LAYERID = 5,      /* int32 */
+    EXACA_FIELD_CELLTYPE_RAW = 6, /* uint8, type | 0x10 for cell-centred octahedra */
+    EXACA_FIELD_NUCLEI_LOC = 7,   /* int32 per nucleation site (NucleiLocation), host state only */
+    EXACA_FIELD_NUCLEI_TIME = 8,  /* int32 per nucleation site (NucleationTimes), host state only */
+};
+
+/* Replaces the initialisation half of RunProgram_Reduced (src/runCA.cpp:37-256): reads the deck
+ * (InputReadFromFile, src/CAinitialize.cpp:232), builds the np = 1 initial state (TempInit_*,
+ * OrientationInit, SubstrateInit_*, ActiveCellWallInit, GrainInit, NucleiInit) on the host and
+ * uploads it. data_dir holds Materials/ and Substrate/ (NULL: the library's data directory). */
+int exaca_create(const char *input_file, const char *data_dir, int verbose, exaca_state **out);
+
+/* Nucleation (src/CAupdate.hpp:15 / src/CAupdate.cpp:67) for time step `cycle` of `layer`.
+ * The nucleation count (the reference's nn) accumulates on the device; see exaca_query. */
+int exaca_nucleation(exaca_state *h, int cycle, int layer);
+
+/* CellCapture (src/CAupdate.hpp:21 / src/CAupdate.cpp:204) for the same cycle. Asynchronous:
+ * unlike the reference (deep_copy of numSteer, src/CAupdate.cpp:243) nothing is read back. */
+int exaca_cell_capture(exaca_state *h, int cycle, int layer);
+
+/* IntermediateOutputAndCheck (src/CAupdate.hpp:31 / src/CAupdate.cpp:732): reduces the five
+ * cell counts, sets *xswitch when no liquid is left in the layer, and for spot-melt problems may
+ * move *cycle forward to the step before the next liquidus crossing. counts may be NULL. */
+int exaca_check(exaca_state *h, int *cycle, int layer, uint64_t counts[5], int *xswitch, int verbose);
+
+/* The layer's do-while loop of RunProgram_Reduced (src/runCA.cpp:290-342) on the device:
+ * Nucleation + CellCapture every step (replayed from captured HIP graphs), the check every 1000
+ * steps. Stops at completion or after max_cycles steps (<= 0: no limit). *cycle receives the
+ * final cycle; *gpu_ms (may be NULL) the device time of the loop. */
+int exaca_run_layer(exaca_state *h, int layer, int max_cycles, int verbose, int *cycle, double *gpu_ms);
+
+/* Device -> host copy of one whole-domain field (kind: enum exaca_field) into dst of
+ * dst_bytes bytes. Replaces the deep_copy at src/runCA.cpp:423-424. */
+int exaca_download(exaca_state *h, int field, void *dst, size_t dst_bytes);
+
+/* Sizes and counters. out[0..15]: X, Y, nz, nzActive, nx, ny, nuclei, last cycle,
+ * nucleation events (nn), sum of frontier sizes, captures, solidifications, waiting-entry steps,
+ * frontier list length, frontier list capacity, device bytes allocated. */
+int exaca_query(const exaca_state *h, int64_t out[16]);
+
+/* Releases the handle and its device memory. */
+int exaca_destroy(exaca_state *h);
+
+/* RunProgram_Reduced(int id, int np, std::string InputFile) (src/runCA.hpp:11): the whole
+ * program for one GPU -- read deck, initialise, run, write <base>.vtk, <base>_Misorientations.vtk
+ * and <base>.log. np must be 1 (one GPU holds the whole domain). */
+int exaca_run_program(int id, int np, const char *input_file);
+
+/* Host-only view of the initial state exaca_create would upload (no device needed): the
+ * reference's host arrays after NucleiInit (src/runCA.cpp:90-195). dims[0..7]: X, Y, nz, nzActive,
+ * nx, ny, nucleation sites, initially active cells. With dst == NULL only dims is filled; field
+ * is one of CELLTYPE, GRAINID, CRITTIMESTEP, LAYERID, NUCLEI_LOC, NUCLEI_TIME. */
+int exaca_host_state(const char *input_file, const char *data_dir, int field, void *dst, size_t dst_bytes,
+                     int64_t dims[8]);
+
+/* Message of the last failure on this thread. */
+const char *exaca_last_error(void);
+
+/* Self test of the device capture geometry (no reference counterpart): n rows of
+ * (orientation, cxold, cyold, czold, xp, yp, zp) -> n rows of (cx, cy, cz, dl, cdl[26]),
+ * using the orientation table of h. in/out are host arrays of 7n / 30n doubles. */
+int exaca_geometry_probe(exaca_state *h, const double *in, double *out, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,141 @@
+"""GPU parity: the HIP path, called through the C ABI, against the oracle on the same decks.
+
+Integer fields (CellType, GrainID), cycle counts and nucleation counts must match exactly; the
+octahedron state (DiagonalLength, DOCenter) is compared bit for bit as float32.
+"""
+import os
+import shutil
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import exaca_amd as ea
+import oracle as orc
+from conftest import DATA, EXAMPLES, c_deck, s_deck
+
+pytestmark = pytest.mark.gpu
+
+ACTIVE = 2
+
+
+def _compare_state(sol, osim, tag, check_octahedra=True):
+    ct_g = sol.field(ea.FIELD_CELLTYPE)
+    ct_o = osim.field(orc.CELLTYPE)
+    bad = np.flatnonzero(ct_g != ct_o)
+    assert bad.size == 0, f"{tag}: CellType differs at {bad.size} cells, first {bad[:5]} gpu {ct_g[bad[:5]]} oracle {ct_o[bad[:5]]}"
+    gid_g = sol.field(ea.FIELD_GRAINID)
+    gid_o = osim.field(orc.GRAINID)
+    bad = np.flatnonzero(gid_g != gid_o)
+    assert bad.size == 0, f"{tag}: GrainID differs at {bad.size} cells, first {bad[:5]}"
+    if check_octahedra:
+        act = np.flatnonzero(ct_o == ACTIVE)
+        d = osim.dims()
+        act = act[act < d["X"] * d["Y"] * d["nzA"]]
+        dl_g = sol.field(ea.FIELD_DIAGONAL)[act]
+        dl_o = osim.field(orc.DIAGLEN)[act]
+        assert np.array_equal(dl_g.view(np.uint32), dl_o.view(np.uint32)), f"{tag}: DiagonalLength differs"
+        c_g = sol.field(ea.FIELD_DOCENTER).reshape(-1, 3)[act]
+        c_o = osim.field(orc.DOCENTER).reshape(-1, 3)[act]
+        assert np.array_equal(c_g.view(np.uint32), c_o.view(np.uint32)), f"{tag}: DOCenter differs"
+
+
+def test_capture_geometry_bitwise(tmp_path):
+    """New octahedron centre, diagonal and all 26 critical lengths for random captures."""
+    deck = c_deck(tmp_path / "g.txt", 6, density=0)
+    guv = np.loadtxt(DATA / "Substrate" / "GrainOrientationVectors_Robert.csv", delimiter=",", skiprows=1,
+                     dtype=np.float64).astype(np.float32)
+    rng = np.random.default_rng(7)
+    n = 4000
+    rows = np.zeros((n, 7))
+    rows[:, 0] = rng.integers(0, 10000, n)
+    cell = rng.integers(1, 200, (n, 3))
+    jitter = rng.uniform(-0.7, 0.7, (n, 3))
+    cold = (cell + 0.5 + jitter).astype(np.float32).astype(np.float64)
+    nb = np.array([[0, -1, 0], [1, -1, 0], [-1, -1, 0], [0, -1, 1], [0, -1, -1], [-1, -1, 1], [1, -1, 1],
+                   [-1, -1, -1], [1, -1, -1], [0, 0, 1], [0, 0, -1], [1, 0, 1], [-1, 0, 1], [1, 0, -1],
+                   [-1, 0, -1], [1, 0, 0], [-1, 0, 0], [0, 1, 0], [1, 1, 0], [-1, 1, 0], [0, 1, 1], [0, 1, -1],
+                   [1, 1, 1], [-1, 1, 1], [1, 1, -1], [-1, 1, -1]])
+    rows[:, 1:4] = cold
+    rows[:, 4:7] = cell + nb[rng.integers(0, 26, n)] + 0.5
+    rows[:8, 1:4] = cell[:8] + 0.5  # cell-centred capturers (nuclei / substrate)
+    with ea.Solver(deck) as sol:
+        out = sol.geometry_probe(rows)
+    for r in range(n):
+        o = int(rows[r, 0])
+        centre, dl, cdl = orc.capture_octahedron(guv.reshape(-1, 9)[o], rows[r, 1:4], rows[r, 4:7])
+        assert np.array_equal(out[r, 0:3], centre), f"row {r}: centre {out[r, 0:3]} vs {centre}"
+        assert np.float32(out[r, 3]) == np.float32(dl), f"row {r}: diagonal"
+        assert np.array_equal(out[r, 4:30].astype(np.float32), cdl), f"row {r}: critical lengths"
+
+
+def test_stepwise_small_domain(tmp_path):
+    """Every one of the first 400 steps of a dense-nucleation 14^3 domain, state compared each step."""
+    deck = c_deck(tmp_path / "s.txt", 14, density=3000, fraction=0.05)
+    osim = orc.OracleSim(deck, DATA)
+    with ea.Solver(deck) as sol:
+        assert sol.query()["nuclei"] == osim.dims()["nuclei"] > 0
+        for cycle in range(1, 401):
+            sol.step(cycle)
+            osim.step(cycle)
+            if cycle % 20 == 0 or cycle < 30:
+                _compare_state(sol, osim, f"cycle {cycle}")
+        q = sol.query()
+        d = osim.dims()
+        assert q["nn"] == d["nn"]
+        assert q["frontier"] == d["frontier"]
+        assert q["captures"] == d["captures"]
+        assert q["solidified"] == d["solidified"]
+
+
+CASES = {
+    "dirs_50_substrate_only": lambda p: EXAMPLES / "Inp_SmallDirSolidification.txt",
+    "dirs_100_baseline": lambda p: EXAMPLES / "Inp_DirSolidification_100.txt",
+    "dirs_40_dense_nuclei": lambda p: c_deck(p / "d.txt", 40, density=400, fraction=0.02),
+    "dirs_ragged_37x23x31": lambda p: c_deck(p / "r.txt", 31, nx=37, ny=23, density=200, fraction=0.1),
+    "spots_2x2": lambda p: s_deck(p / "s.txt", nsx=2, nsy=2, offset=30, radius=20, density=60),
+    "spots_1x2_jumps": lambda p: EXAMPLES / "Inp_SmallSpotMelt.txt",
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_full_run_matches_oracle(case, tmp_path):
+    deck = CASES[case](tmp_path)
+    osim = orc.OracleSim(deck, DATA)
+    ocycle, _ = osim.run()
+    with ea.Solver(deck) as sol:
+        gcycle, _ = sol.run()
+        q = sol.query()
+        assert gcycle == ocycle, f"final cycle gpu {gcycle} oracle {ocycle}"
+        d = osim.dims()
+        assert q["nn"] == d["nn"]
+        assert q["captures"] == d["captures"]
+        assert q["solidified"] == d["solidified"]
+        assert q["frontier"] == d["frontier"]
+        _compare_state(sol, osim, case)
+
+
+def test_drop_in_outputs_identical(tmp_path):
+    """RunProgram_Reduced drop-in: vtk files byte-identical to the oracle's, log identical apart
+    from the timing lines."""
+    deck = c_deck(tmp_path / "io.txt", 30, density=300, fraction=0.05, printing="Y", base="io",
+                  out=str(tmp_path / "gpu") + "/")
+    (tmp_path / "gpu").mkdir()
+    ea.run_program(deck)
+    odeck = c_deck(tmp_path / "io_oracle.txt", 30, density=300, fraction=0.05, printing="Y", base="io",
+                   out=str(tmp_path / "cpu") + "/")
+    (tmp_path / "cpu").mkdir()
+    osim = orc.OracleSim(odeck, DATA)
+    osim.run()
+    osim.write_outputs()
+    for name in ["io.vtk", "io_Misorientations.vtk"]:
+        a = (tmp_path / "gpu" / name).read_bytes()
+        b = (tmp_path / "cpu" / name).read_bytes()
+        assert a == b, f"{name} differs"
+    la = (tmp_path / "gpu" / "io.log").read_text().splitlines()
+    lb = (tmp_path / "cpu" / "io.log").read_text().splitlines()
+    assert len(la) == len(lb)
+    for x, y in zip(la, lb):
+        if "time" in x or "took" in x or "input file" in x:
+            continue
+        assert x == y
