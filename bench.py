@@ -1,0 +1,161 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X CA time step on the 800^3 directional-solidification workload.
+
+A "step" is one CA time step (Nucleation + CellCapture, one cycle of the reference's loop,
+src/runCA.cpp:290-342) over the whole 804 x 804 x 802-cell domain. After --warmup untimed steps,
+exactly --steps steps run from the device-resident state (HIP graphs, termination check every
+1000 steps as in the reference); value = active-cell updates per second, i.e. the sum over the
+timed steps of the frontier size (cells past their liquidus time, the reference's numSteer)
+divided by the device time of those steps.
+
+roofline: the dominant kernel pair is CellCapture (k_claim + k_apply). Its algorithmic bytes are
+137 B per active-cell update + 125 B per capture + 1 B per solidification (the reference layout's
+bytes: own state, the 26 critical lengths and the new cell's octahedron), divided by the kernels'
+event-timed durations over --profile-steps further steps.
+
+cpu_baseline: the oracle (the serial C restatement of the reference's algorithm) on a bounded
+sample of the same workload: the first --cpu-cycles steps of the same deck cut to its bottom
+--cpu-z planes (the random streams are drawn plane by plane, so the substrate, nuclei and their
+times in those planes are the 800^3 deck's own); its CellCapture time as in the reference's log.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+from pathlib import Path
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO))
+
+METRIC = "active-cell updates/s + HBM GB/s at 800³; ensemble solves/s, p99 vs plain loop"
+HBM_PEAK_GBPS = 8000.0
+
+
+def cpu_baseline(deck: Path, z: int, cycles: int) -> dict:
+    sys.path.insert(0, str(REPO / "oracle"))
+    import oracle as orc
+    import exaca_amd as ea
+
+    text = deck.read_text().splitlines()
+    text = [f"Domain size in z: {z}" if ln.startswith("Domain size in z") else ln for ln in text]
+    with tempfile.TemporaryDirectory() as d:
+        sample = Path(d) / "sample.txt"
+        sample.write_text("\n".join(text) + "\n")
+        sim = orc.OracleSim(sample, ea.DATA_DIR)
+        cycle, cap_s = sim.run(max_cycles=cycles)
+        dims = sim.dims()
+        sim.close()
+    return {
+        "value": dims["frontier"] / cap_s,
+        "unit": "active-cell updates/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"first {cycle} steps of {deck.name} cut to its bottom {z} planes "
+                  f"({dims['X']}x{dims['Y']}x{dims['nz']} cells, same random streams); serial C oracle, "
+                  f"CellCapture time {cap_s:.2f} s for {dims['frontier']} active-cell updates",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=6000)
+    ap.add_argument("--warmup", type=int, default=1000)
+    ap.add_argument("--deck", default=str(REPO / "examples" / "Inp_DirSolidification_800.txt"))
+    ap.add_argument("--profile-steps", type=int, default=400)
+    ap.add_argument("--cpu-z", type=int, default=40)
+    ap.add_argument("--cpu-cycles", type=int, default=150)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs one GPU: the whole CA domain lives on a single MI355X (asked for {args.gpus})",
+              file=sys.stderr)
+        return 2
+
+    import exaca_amd as ea
+
+    deck = Path(args.deck)
+    t0 = time.perf_counter()
+    sol = ea.Solver(deck)
+    init_s = time.perf_counter() - t0
+    print(f"[bench] initialised {deck.name}: {sol.X}x{sol.Y}x{sol.nz} cells in {init_s:.1f} s", file=sys.stderr,
+          flush=True)
+    copy_bytes = 1 << 32
+    copy_ms = ea.copy_probe(copy_bytes, 10)
+    copy_gbps = 2 * copy_bytes / (copy_ms / 1e3) / 1e9
+
+    if args.warmup > 0:
+        sol.run(max_cycles=args.warmup)
+    q0 = sol.query()
+    t1 = time.perf_counter()
+    cycle, gpu_ms = sol.run(max_cycles=q0["cycle"] + args.steps)
+    wall = time.perf_counter() - t1
+    q1 = sol.query()
+    steps = cycle - q0["cycle"]
+    frontier = q1["frontier"] - q0["frontier"]
+    print(f"[bench] timed {steps} steps in {gpu_ms:.1f} ms device / {wall:.3f} s wall; "
+          f"{frontier} active-cell updates, {q1['captures'] - q0['captures']} captures", file=sys.stderr, flush=True)
+    value = frontier / (gpu_ms / 1e3)
+
+    prof = sol.profile_steps(args.profile_steps)
+    cap_ms = prof["claim_ms"] + prof["apply_ms"]
+    launches = max(prof["launches"], 1)
+    bytes_alg = 137 * prof["frontier"] + 125 * prof["captures"] + 1 * prof["solidified"]
+    achieved = bytes_alg / (cap_ms / 1e3) / 1e9 if cap_ms > 0 else 0.0
+    sol.close()
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "active-cell updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": gpu_ms / max(steps, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64+f32 (reference types per quantity)",
+        "data": "synthetic: the deck's own seeded substrate, nuclei and orientations",
+        "config": {
+            "workload": "directional solidification 800^3 (G = R = 1e6, N = 10, substrate 1/25 um^2, "
+                        "nuclei 1e13 m^-3), cycles "
+                        f"{q0['cycle'] + 1}-{cycle}",
+            "deck": deck.name,
+            "cells": sol.ncell,
+            "nuclei": q1["nuclei"],
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "CellCapture = k_claim + k_apply",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBPS,
+            "traffic": None,
+            "avg_launch_us": 1e3 * cap_ms / launches,
+            "profiled_steps": prof["steps"],
+            "bytes_per_update": 137,
+        },
+        "kernel_avg_us": {
+            "k_nucleate": 1e3 * prof["nucleation_ms"] / launches,
+            "k_claim": 1e3 * prof["claim_ms"] / launches,
+            "k_apply": 1e3 * prof["apply_ms"] / launches,
+        },
+        "copy_GBps": copy_gbps,
+        "init_s": init_s,
+        "wall_s": wall,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(deck, args.cpu_z, args.cpu_cycles)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -46,6 +46,9 @@ _SIGNATURES = {
                                         ctypes.c_size_t, _I64P]),
     "exaca_last_error": (ctypes.c_char_p, []),
     "exaca_geometry_probe": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
+    "exaca_profile_steps": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                           ctypes.POINTER(ctypes.c_double), _I64P]),
+    "exaca_copy_probe": (ctypes.c_int, [ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
 }
 
 _lib = None
@@ -102,6 +105,13 @@ def host_state(deck, field: int | None = None, data_dir=None):
 def run_program(deck) -> None:
     """RunProgram_Reduced(0, 1, deck): the whole drop-in program on the current GPU."""
     _ok(library().exaca_run_program(0, 1, _b(deck)))
+
+
+def copy_probe(nbytes: int, reps: int = 20) -> float:
+    """Average device time (ms) of a plain copy of nbytes (HBM calibration)."""
+    ms = ctypes.c_double(0)
+    _ok(library().exaca_copy_probe(nbytes, reps, ctypes.byref(ms)))
+    return ms.value
 
 
 class Solver:
@@ -174,6 +184,14 @@ class Solver:
         keys = ["X", "Y", "nz", "nzA", "nx", "ny", "nuclei", "cycle", "nn", "frontier", "captures",
                 "solidified", "waiting", "list_len", "list_cap", "device_bytes"]
         return dict(zip(keys, list(out)))
+
+    def profile_steps(self, nsteps: int, layer: int = 0) -> dict:
+        """Runs nsteps more steps with per-kernel HIP events; summed kernel times in ms."""
+        ms = (ctypes.c_double * 3)()
+        cnt = (ctypes.c_int64 * 5)()
+        _ok(self._lib.exaca_profile_steps(self._h, layer, nsteps, ms, cnt))
+        return {"nucleation_ms": ms[0], "claim_ms": ms[1], "apply_ms": ms[2], "frontier": cnt[0],
+                "captures": cnt[1], "solidified": cnt[2], "launches": cnt[3], "steps": cnt[4]}
 
     def geometry_probe(self, rows: np.ndarray) -> np.ndarray:
         rows = np.ascontiguousarray(rows, dtype=np.float64).reshape(-1, 7)

@@ -81,6 +81,7 @@ struct DomainView {
     unsigned long long *win;
     uint32_t *list[2];
     uint32_t *mask;
+    uint32_t *slice_out;    // next-frontier entries produced by each workgroup slice
     uint32_t list_cap;
     int32_t nnuclei;
     const int32_t *nloc;

@@ -202,10 +202,12 @@ __device__ __forceinline__ unsigned long long block_sum(unsigned long long v, un
 // Contiguous slice of the frontier for this workgroup. Workgroups are dispatched round-robin over
 // the 8 XCDs, so consecutive slices are given to workgroups on the same XCD: neighbouring frontier
 // cells (and their neighbour reads and claims) then share one L2.
-__device__ __forceinline__ void frontier_slice(uint32_t count, uint32_t cap, uint32_t &begin, uint32_t &end) {
+__device__ __forceinline__ void frontier_slice(uint32_t count, uint32_t cap, uint32_t &begin, uint32_t &end,
+                                               uint32_t *slice_out = nullptr) {
     const uint32_t n = count < cap ? count : cap;  // an overflowed list is flagged, never over-read
     const uint32_t nb = gridDim.x, b = blockIdx.x;
     const uint32_t logical = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
+    if (slice_out) *slice_out = logical;
     uint32_t chunk = (n + nb - 1) / nb;
     chunk = (chunk + kBlock - 1) / kBlock * kBlock;
     const unsigned long long b0 = static_cast<unsigned long long>(logical) * chunk;
@@ -300,60 +302,43 @@ __global__ __launch_bounds__(kBlock) void k_claim(DomainView d, int p, int layer
     if (threadIdx.x == 0 && tot) atomicAdd(&d.stats->frontier, tot);
 }
 
+// Pass B. Per frontier entry the outcome is written back over its mask as (keep << 30 | captured
+// neighbours); every workgroup stores the number of next-frontier entries its slice produces so
+// that k_emit can lay the next frontier out in the same order as the current one.
 __global__ __launch_bounds__(kBlock) void k_apply(DomainView d, int p, int layer) {
     __shared__ unsigned long long sh[kBlock / 64];
     StepCtl *ctl = d.ctl;
     const unsigned long long stamp = static_cast<unsigned long long>(0xFFFFFFFFu - ctl->serial[p]) << 32;
-    uint32_t begin, end;
-    frontier_slice(ctl->count[p], d.list_cap, begin, end);
+    uint32_t begin, end, slice;
+    frontier_slice(ctl->count[p], d.list_cap, begin, end, &slice);
     const uint32_t *list = d.list[p];
-    uint32_t *next = d.list[1 - p];
-    unsigned long long ncap = 0, nsol = 0, nwait = 0;
-    // All lanes of a wave run the same number of iterations so the wave-wide scan stays converged.
-    const uint32_t wave_begin = begin + (threadIdx.x & ~63u);
-    for (uint32_t base = wave_begin; base < end; base += kBlock) {
-        const uint32_t i = base + (threadIdx.x & 63u);
-        uint32_t j = 0, captured = 0, keep = 0;
-        if (i < end) {
-            j = list[i];
-            const uint32_t m = d.mask[i];
-            if (m == kWaitMask) {
-                keep = 1;
-                nwait++;
-            }
-            else {
-                uint32_t lcount = 0;
-                for (uint32_t r = m; r; r &= r - 1) {
-                    const int l = __builtin_ctz(r);
-                    const unsigned long long w = d.win[nb_index(j, l, d.X, d.Y)];
-                    if ((w & 0xFFFFFFFF00000000ull) == stamp) {
-                        const uint32_t wj = static_cast<uint32_t>(w);
-                        lcount |= (wj >= j);
-                        if (wj == j) captured |= 1u << l;
-                    }
-                    else {
-                        lcount = 1;
-                    }
+    unsigned long long ncap = 0, nsol = 0, nwait = 0, produced = 0;
+    for (uint32_t i = begin + threadIdx.x; i < end; i += kBlock) {
+        const uint32_t j = list[i];
+        const uint32_t m = d.mask[i];
+        uint32_t captured = 0, keep = 0;
+        if (m == kWaitMask) {
+            keep = 1;
+            nwait++;
+        }
+        else {
+            for (uint32_t r = m; r; r &= r - 1) {
+                const int l = __builtin_ctz(r);
+                const unsigned long long w = d.win[nb_index(j, l, d.X, d.Y)];
+                if ((w & 0xFFFFFFFF00000000ull) == stamp) {
+                    const uint32_t wj = static_cast<uint32_t>(w);
+                    keep |= (wj >= j);
+                    if (wj == j) captured |= 1u << l;
                 }
-                keep = lcount;
-                if (!lcount) {
-                    d.ct[j] = kSolid;
-                    nsol++;
+                else {
+                    keep = 1;  // nobody reached it this step: still liquid for this cell
                 }
             }
+            if (!keep) {
+                d.ct[j] = kSolid;
+                nsol++;
+            }
         }
-        const uint32_t mine = keep + __builtin_popcount(captured);
-        uint32_t total;
-        const uint32_t incl = wave_inclusive_scan(mine, total);
-        uint32_t wbase = 0;
-        if ((threadIdx.x & 63) == 63 && total) wbase = atomicAdd(&ctl->count[1 - p], total);
-        wbase = __shfl(wbase, 63, 64);
-        if (static_cast<unsigned long long>(wbase) + total > d.list_cap) {
-            if ((threadIdx.x & 63) == 0) atomicOr(&ctl->error, 1);
-            continue;
-        }
-        uint32_t pos = wbase + incl - mine;
-        if (keep) next[pos++] = j;
         if (captured) {
             const int h = d.gid[j];
             int o = orientation_of(h);
@@ -379,17 +364,79 @@ __global__ __launch_bounds__(kBlock) void k_apply(DomainView d, int p, int layer
                 d.cy[q] = oc.cy;
                 d.cz[q] = oc.cz;
                 d.ct[q] = kActive;
-                next[pos++] = q;
                 ncap++;
             }
         }
+        d.mask[i] = (keep << 30) | captured;
+        produced += keep + __builtin_popcount(captured);
     }
-    unsigned long long t = block_sum(ncap, sh);
+    unsigned long long t = block_sum(produced, sh);
+    if (threadIdx.x == 0) d.slice_out[slice] = static_cast<uint32_t>(t);
+    t = block_sum(ncap, sh);
     if (threadIdx.x == 0 && t) atomicAdd(&d.stats->captures, t);
     t = block_sum(nsol, sh);
     if (threadIdx.x == 0 && t) atomicAdd(&d.stats->solidified, t);
     t = block_sum(nwait, sh);
     if (threadIdx.x == 0 && t) atomicAdd(&d.stats->waiting, t);
+}
+
+// Writes the next frontier: slice s's entries start after those of slices 0 .. s-1, and within a
+// slice each entry's survivor comes first, then its captured neighbours in neighbour order. The
+// front therefore keeps the spatial order of the current list (a column-ordered front stays
+// column-ordered), which keeps the per-cell reads of the next step coalesced.
+__global__ __launch_bounds__(kBlock) void k_emit(DomainView d, int p) {
+    __shared__ unsigned long long sh[kBlock / 64];
+    __shared__ uint32_t wave_tot[kBlock / 64];
+    StepCtl *ctl = d.ctl;
+    uint32_t begin, end, slice;
+    frontier_slice(ctl->count[p], d.list_cap, begin, end, &slice);
+    unsigned long long before = 0, all = 0;
+    for (uint32_t s = threadIdx.x; s < gridDim.x; s += kBlock) {
+        const uint32_t v = d.slice_out[s];
+        all += v;
+        if (s < slice) before += v;
+    }
+    before = block_sum(before, sh);
+    all = block_sum(all, sh);
+    __shared__ unsigned long long base_sh;
+    if (threadIdx.x == 0) {
+        base_sh = before;
+        if (slice == 0) {
+            ctl->count[1 - p] = static_cast<uint32_t>(all < d.list_cap ? all : d.list_cap);
+            if (all > d.list_cap) atomicOr(&ctl->error, 1);
+        }
+    }
+    __syncthreads();
+    unsigned long long base = base_sh;
+    const uint32_t *list = d.list[p];
+    uint32_t *next = d.list[1 - p];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (uint32_t i0 = begin; i0 < end; i0 += kBlock) {
+        const uint32_t i = i0 + threadIdx.x;
+        uint32_t j = 0, m = 0;
+        if (i < end) {
+            j = list[i];
+            m = d.mask[i];
+        }
+        const uint32_t keep = (m >> 30) & 1u, captured = m & ((1u << 26) - 1);
+        const uint32_t mine = keep + __builtin_popcount(captured);
+        uint32_t wtot;
+        const uint32_t incl = wave_inclusive_scan(mine, wtot);
+        if (lane == 63) wave_tot[wave] = wtot;
+        __syncthreads();
+        uint32_t wbase = 0, round = 0;
+        for (int w = 0; w < kBlock / 64; w++) {
+            if (w < wave) wbase += wave_tot[w];
+            round += wave_tot[w];
+        }
+        unsigned long long pos = base + wbase + incl - mine;
+        if (pos + mine <= d.list_cap) {
+            if (keep) next[pos++] = j;
+            for (uint32_t r = captured; r; r &= r - 1) next[pos++] = nb_index(j, __builtin_ctz(r), d.X, d.Y);
+        }
+        base += round;
+        __syncthreads();
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_check(DomainView d, int cycle, int layer, CheckResult *out) {
@@ -466,6 +513,12 @@ __global__ __launch_bounds__(kBlock) void k_geometry_probe(const float *guv, con
     for (int l = 0; l < 26; l++) b[4 + l] = critical_length(f, a[4], a[5], a[6], o.cx, o.cy, o.cz, l);
 }
 
+__global__ __launch_bounds__(kBlock) void k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n) {
+    for (size_t i = static_cast<size_t>(blockIdx.x) * kBlock + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * kBlock)
+        dst[i] = src[i];
+}
+
 inline int grid_for(long long n, int cap) {
     long long b = (n + kBlock - 1) / kBlock;
     if (b < 1) b = 1;
@@ -482,12 +535,16 @@ void launch_claim(const DomainView &d, int parity, int layer, int blocks, hipStr
 }
 void launch_apply(const DomainView &d, int parity, int layer, int blocks, hipStream_t s) {
     k_apply<<<blocks, kBlock, 0, s>>>(d, parity, layer);
+    k_emit<<<blocks, kBlock, 0, s>>>(d, parity);
 }
 void launch_check(const DomainView &d, int cycle, int layer, CheckResult *out, hipStream_t s) {
     k_check<<<grid_for(d.ncell, 4096), kBlock, 0, s>>>(d, cycle, layer, out);
 }
 void launch_seed_init(const DomainView &d, const uint32_t *cells, uint32_t n, hipStream_t s) {
     k_seed_init<<<grid_for(n, 1024), kBlock, 0, s>>>(d, cells, n);
+}
+void launch_copy(const uint4 *src, uint4 *dst, size_t n, hipStream_t s) {
+    k_copy<<<grid_for(static_cast<long long>(n), 1 << 16), kBlock, 0, s>>>(src, dst, n);
 }
 void launch_geometry_probe(const float *guv, const double *in, double *out, int n, hipStream_t s) {
     k_geometry_probe<<<grid_for(n, 1 << 20), kBlock, 0, s>>>(guv, in, out, n);

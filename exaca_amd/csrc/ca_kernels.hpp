@@ -19,6 +19,8 @@ void launch_apply(const DomainView &d, int parity, int layer, int blocks, hipStr
 void launch_check(const DomainView &d, int cycle, int layer, CheckResult *out, hipStream_t s);
 // Octahedra of the initially active cells (src/CAinitialize.cpp:1936-2043).
 void launch_seed_init(const DomainView &d, const uint32_t *cells, uint32_t n, hipStream_t s);
+// Plain copy of n 16-byte words (HBM calibration).
+void launch_copy(const uint4 *src, uint4 *dst, size_t n, hipStream_t s);
 // Device-side self test of the capture geometry against host values: one octahedron per input row.
 void launch_geometry_probe(const float *guv, const double *in, double *out, int n, hipStream_t s);
 

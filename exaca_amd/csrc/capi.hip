@@ -3,6 +3,7 @@
 // drop-in.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdlib>
 #include <memory>
@@ -132,6 +133,7 @@ void build_device_state(exaca_state *h) {
     d.list[0] = h->alloc<uint32_t>(n);
     d.list[1] = h->alloc<uint32_t>(n);
     d.mask = h->alloc<uint32_t>(n);
+    d.slice_out = h->alloc<uint32_t>(h->blocks);
     d.nnuclei = static_cast<int32_t>(hd.nloc.size());
     int32_t *nloc = h->alloc<int32_t>(hd.nloc.size());
     int32_t *ntime = h->alloc<int32_t>(hd.ntime.size());
@@ -475,6 +477,89 @@ int exaca_query(const exaca_state *h, int64_t out[16]) {
 
 int exaca_destroy(exaca_state *h) {
     return guarded([&] { delete h; });
+}
+
+int exaca_profile_steps(exaca_state *h, int layer, int nsteps, double ms[3], int64_t counts[5]) {
+    return guarded([&] {
+        if (!h || !ms || !counts || nsteps < 0) throw std::invalid_argument("bad argument");
+        if (h->pending >= 0) throw StateError("profile_steps called between nucleation and capture");
+        exaca::StepStats before, after;
+        HIP_OK(hipStreamSynchronize(h->stream));
+        HIP_OK(hipMemcpy(&before, h->stats, sizeof before, hipMemcpyDeviceToHost));
+        const int per_batch = 256;
+        std::vector<hipEvent_t> ev(6 * per_batch);
+        for (auto &e : ev) HIP_OK(hipEventCreate(&e));
+        double acc[3] = {0, 0, 0};
+        int done = 0, xswitch = 0;
+        while (done < nsteps && !xswitch) {
+            const int batch = std::min(per_batch, nsteps - done);
+            int k = 0;
+            for (; k < batch; k++) {
+                hipEvent_t *e = &ev[6 * k];
+                HIP_OK(hipEventRecord(e[0], h->stream));
+                step_nucleation(h, layer);
+                HIP_OK(hipEventRecord(e[1], h->stream));
+                HIP_OK(hipEventRecord(e[2], h->stream));
+                exaca::launch_claim(h->dv, h->parity, layer, h->blocks, h->stream);
+                HIP_OK(hipEventRecord(e[3], h->stream));
+                HIP_OK(hipEventRecord(e[4], h->stream));
+                exaca::launch_apply(h->dv, h->parity, layer, h->blocks, h->stream);
+                HIP_OK(hipEventRecord(e[5], h->stream));
+                h->parity ^= 1;
+                h->cycle += 1;
+                if (h->cycle % 1000 == 0) {
+                    k++;
+                    break;
+                }
+            }
+            HIP_OK(hipStreamSynchronize(h->stream));
+            for (int q = 0; q < k; q++)
+                for (int r = 0; r < 3; r++) {
+                    float t = 0;
+                    HIP_OK(hipEventElapsedTime(&t, ev[6 * q + 2 * r], ev[6 * q + 2 * r + 1]));
+                    acc[r] += t;
+                }
+            done += k;
+            if (h->cycle % 1000 == 0) {
+                int c = h->cycle;
+                device_check(h, &c, layer, nullptr, &xswitch, 0);
+            }
+        }
+        for (auto &e : ev) (void)hipEventDestroy(e);
+        HIP_OK(hipMemcpy(&after, h->stats, sizeof after, hipMemcpyDeviceToHost));
+        for (int r = 0; r < 3; r++) ms[r] = acc[r];
+        counts[0] = static_cast<int64_t>(after.frontier - before.frontier);
+        counts[1] = static_cast<int64_t>(after.captures - before.captures);
+        counts[2] = static_cast<int64_t>(after.solidified - before.solidified);
+        counts[3] = done;
+        counts[4] = done;
+    });
+}
+
+int exaca_copy_probe(size_t bytes, int reps, double *ms) {
+    return guarded([&] {
+        if (!ms || reps <= 0 || bytes < 16) throw std::invalid_argument("bad argument");
+        const size_t n = bytes / 16;
+        uint4 *a = nullptr, *b = nullptr;
+        HIP_OK(hipMalloc(&a, n * 16));
+        HIP_OK(hipMalloc(&b, n * 16));
+        HIP_OK(hipMemset(a, 1, n * 16));
+        hipEvent_t e0, e1;
+        HIP_OK(hipEventCreate(&e0));
+        HIP_OK(hipEventCreate(&e1));
+        exaca::launch_copy(a, b, n, nullptr);
+        HIP_OK(hipEventRecord(e0, nullptr));
+        for (int r = 0; r < reps; r++) exaca::launch_copy(a, b, n, nullptr);
+        HIP_OK(hipEventRecord(e1, nullptr));
+        HIP_OK(hipEventSynchronize(e1));
+        float t = 0;
+        HIP_OK(hipEventElapsedTime(&t, e0, e1));
+        *ms = t / reps;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipFree(a);
+        (void)hipFree(b);
+    });
 }
 
 int exaca_geometry_probe(exaca_state *h, const double *in, double *out, int n) {

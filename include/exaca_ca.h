@@ -93,6 +93,17 @@ int exaca_host_state(const char *input_file, const char *data_dir, int field, vo
 /* Message of the last failure on this thread. */
 const char *exaca_last_error(void);
 
+/* Timed stepping (no reference counterpart): runs nsteps further steps of `layer` like
+ * exaca_run_layer, without graphs, with HIP events on the handle's stream around every kernel.
+ * ms[0..2] receive the summed device time of the nucleation, claim and apply kernels; counts
+ * [0..4] the frontier-size sum, captures, solidifications, kernel launches per kind and the number
+ * of steps executed (checks and cycle jumps are applied as in exaca_run_layer). */
+int exaca_profile_steps(exaca_state *h, int layer, int nsteps, double ms[3], int64_t counts[5]);
+
+/* HBM calibration (no reference counterpart): a plain device copy of `bytes` bytes repeated
+ * `reps` times; *ms receives the average time of one copy. */
+int exaca_copy_probe(size_t bytes, int reps, double *ms);
+
 /* Self test of the device capture geometry (no reference counterpart): n rows of
  * (orientation, cxold, cyold, czold, xp, yp, zp) -> n rows of (cx, cy, cz, dl, cdl[26]),
  * using the orientation table of h. in/out are host arrays of 7n / 30n doubles. */

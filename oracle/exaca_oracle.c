@@ -1156,6 +1156,7 @@ void oracle_check(oracle_sim *s, int *cycle, uint64_t counts[5], int *xswitch, i
         printf("cycle = %d Superheated liquid cells = %lu Undercooled liquid cells = %lu Number of nucleation "
                "events this layer %lld Remaining liquid cells in future layers of this simulation = %lu\n",
                *cycle, (unsigned long)c[0], (unsigned long)c[1], s->nn, (unsigned long)c[4]);
+    if (verbose) fflush(stdout);
     if (c[0] + c[1] == 0) *xswitch = 1;
     if (*xswitch == 0 && strcmp(s->type, "S") == 0 && c[1] == 0) {
         unsigned long next = (unsigned long)-1;
@@ -1167,7 +1168,9 @@ void oracle_check(oracle_sim *s, int *cycle, uint64_t counts[5], int *xswitch, i
         }
         if ((next - (unsigned long)(long)*cycle) > 5000) {
             *cycle = (int)(next - 1);
+            s->cycle = *cycle;
             if (verbose) printf("Jumping to cycle %d\n", *cycle + 1);
+            if (verbose) fflush(stdout);
         }
     }
 }
