@@ -43,6 +43,52 @@ __device__ __forceinline__ int executed_since(int t, int cycle, const StepCtl *c
 struct Coord {
     int x, y, z;
 };
+// Unsigned division by a runtime-uniform divisor as a multiply-high and shifts (the round-up
+// method; exact for every 32-bit dividend). mul/shift come from Divisor::make on the host side of
+// the kernel (uniform, so they live in scalar registers).
+struct Divisor {
+    uint32_t d, mul, sh1, sh2;
+    __host__ __device__ static Divisor make(uint32_t d) {
+        Divisor r;
+        r.d = d;
+        uint32_t l = 0;
+        while ((1ull << l) < d) l++;
+        r.mul = static_cast<uint32_t>(((1ull << 32) * ((1ull << l) - d)) / d + 1);
+        r.sh1 = l > 0 ? 1 : 0;
+        r.sh2 = l > 0 ? l - 1 : 0;
+        return r;
+    }
+    __device__ __forceinline__ uint32_t div(uint32_t n) const {
+        const uint32_t t = __umulhi(n, mul);
+        return (t + ((n - t) >> sh1)) >> sh2;
+    }
+};
+struct Grid {
+    Divisor xy, y;
+    int X, Y, nzA;
+    int off[26];  // linear offsets of the 26 neighbours
+};
+__device__ __forceinline__ Grid grid_of(const DomainView &d) {
+    Grid g;
+    g.X = d.X;
+    g.Y = d.Y;
+    g.nzA = d.nzA;
+    g.xy = Divisor::make(static_cast<uint32_t>(d.X) * static_cast<uint32_t>(d.Y));
+    g.y = Divisor::make(static_cast<uint32_t>(d.Y));
+#pragma unroll
+    for (int l = 0; l < 26; l++) g.off[l] = (kNbZ[l] * d.X + kNbX[l]) * d.Y + kNbY[l];
+    return g;
+}
+__device__ __forceinline__ Coord decode(uint32_t p, const Grid &g) {
+    Coord c;
+    const uint32_t z = g.xy.div(p);
+    const uint32_t rem = p - z * g.xy.d;
+    const uint32_t x = g.y.div(rem);
+    c.z = static_cast<int>(z);
+    c.x = static_cast<int>(x);
+    c.y = static_cast<int>(rem - x * g.y.d);
+    return c;
+}
 __device__ __forceinline__ Coord decode(uint32_t p, int X, int Y) {
     const uint32_t XY = static_cast<uint32_t>(X) * static_cast<uint32_t>(Y);
     Coord c;
@@ -60,6 +106,7 @@ __device__ __forceinline__ uint32_t nb_index(uint32_t p, int l, int X, int Y) {
 // (src/CAupdate.cpp:276-318) and its MyNeighborZ < nzActive test (:328). z >= 0 always holds for
 // an active cell (plane 0 is a wall); it is tested for memory safety.
 __device__ __forceinline__ uint32_t valid_neighbours(const Coord &c, int X, int Y, int nzA) {
+    if (c.x > 0 && c.x < X - 1 && c.y > 0 && c.y < Y - 1 && c.z > 0 && c.z < nzA - 1) return (1u << 26) - 1;
     uint32_t m = 0;
 #pragma unroll
     for (int l = 0; l < 26; l++) {
@@ -243,6 +290,9 @@ __global__ __launch_bounds__(kBlock) void k_nucleate(DomainView d, int p, int la
     if (nn) atomicAdd(&d.stats->nucleated, nn);
 }
 
+// Pass A. All per-cell loads (own state, octahedron, the 26 neighbour types) are issued before any
+// of them is used, so a frontier cell costs one memory round trip; a neighbour outside the slab
+// loads the cell's own (active, never liquid) type instead and is masked out.
 __global__ __launch_bounds__(kBlock) void k_claim(DomainView d, int p, int layer) {
     __shared__ unsigned long long sh[kBlock / 64];
     const StepCtl *ctl = d.ctl;
@@ -252,49 +302,54 @@ __global__ __launch_bounds__(kBlock) void k_claim(DomainView d, int p, int layer
     uint32_t begin, end;
     frontier_slice(ctl->count[p], d.list_cap, begin, end);
     const uint32_t *list = d.list[p];
+    const Grid gr = grid_of(d);
     unsigned long long front = 0;
     for (uint32_t i = begin + threadIdx.x; i < end; i += kBlock) {
         const uint32_t j = list[i];
         const int t = d.cts[j];
-        if (!(cycle > t && d.lid[j] <= layer)) {
+        const int8_t lay = d.lid[j];
+        const float dl0 = d.dl[j];
+        const uint8_t own = d.ct[j];
+        const int h = d.gid[j];
+        const double cx = d.cx[j], cy = d.cy[j], cz = d.cz[j];
+        const Coord c = decode(j, gr);
+        const uint32_t valid = valid_neighbours(c, gr.X, gr.Y, gr.nzA);
+        uint32_t liquid = 0;
+#pragma unroll
+        for (int l = 0; l < 26; l++) {
+            const uint32_t q = ((valid >> l) & 1u) ? j + gr.off[l] : j;
+            liquid |= static_cast<uint32_t>(d.ct[q] == kLiquid) << l;
+        }
+        if (!(cycle > t && lay <= layer)) {
             d.mask[i] = kWaitMask;
             continue;
         }
         front++;
         int k = executed_since(t, cycle, ctl, njumps);
         k = k < 0 ? 0 : (k < d.ngrow - 1 ? k : d.ngrow - 1);
-        const float dlj = static_cast<float>(static_cast<double>(d.dl[j]) + d.grow[k]);
+        const float dlj = static_cast<float>(static_cast<double>(dl0) + d.grow[k]);
         d.dl[j] = dlj;
-        const Coord c = decode(j, d.X, d.Y);
-        const uint32_t valid = valid_neighbours(c, d.X, d.Y, d.nzA);
-        uint32_t liquid = 0;
-#pragma unroll
-        for (int l = 0; l < 26; l++)
-            if ((valid >> l) & 1u)
-                if (d.ct[nb_index(j, l, d.X, d.Y)] == kLiquid) liquid |= 1u << l;
         d.mask[i] = liquid;
         if (!liquid) continue;
-        int o = orientation_of(d.gid[j]);
+        int o = orientation_of(h);
         if (o < 0) {
             atomicOr(&d.ctl->error, 2);
             o = 0;
         }
         const unsigned long long key = stamp | j;
-        if (d.ct[j] & kSeedBit) {
+        if (own & kSeedBit) {
             const float *tab = d.seed_cdl + 26 * o;
             for (uint32_t m = liquid; m; m &= m - 1) {
                 const int l = __builtin_ctz(m);
-                if (dlj >= tab[l]) atomicMin(&d.win[nb_index(j, l, d.X, d.Y)], key);
+                if (dlj >= tab[l]) atomicMin(&d.win[j + gr.off[l]], key);
             }
         }
         else {
             const Faces f = faces_of(d.guv + 9 * o);
             const double xp = (c.x + d.xoff) + 0.5, yp = (c.y + d.yoff) + 0.5, zp = c.z + 0.5;
-            const double cx = d.cx[j], cy = d.cy[j], cz = d.cz[j];
             for (uint32_t m = liquid; m; m &= m - 1) {
                 const int l = __builtin_ctz(m);
-                if (dlj >= critical_length(f, xp, yp, zp, cx, cy, cz, l))
-                    atomicMin(&d.win[nb_index(j, l, d.X, d.Y)], key);
+                if (dlj >= critical_length(f, xp, yp, zp, cx, cy, cz, l)) atomicMin(&d.win[j + gr.off[l]], key);
             }
         }
     }
@@ -312,6 +367,7 @@ __global__ __launch_bounds__(kBlock) void k_apply(DomainView d, int p, int layer
     uint32_t begin, end, slice;
     frontier_slice(ctl->count[p], d.list_cap, begin, end, &slice);
     const uint32_t *list = d.list[p];
+    const Grid gr = grid_of(d);
     unsigned long long ncap = 0, nsol = 0, nwait = 0, produced = 0;
     for (uint32_t i = begin + threadIdx.x; i < end; i += kBlock) {
         const uint32_t j = list[i];
@@ -322,17 +378,17 @@ __global__ __launch_bounds__(kBlock) void k_apply(DomainView d, int p, int layer
             nwait++;
         }
         else {
-            for (uint32_t r = m; r; r &= r - 1) {
-                const int l = __builtin_ctz(r);
-                const unsigned long long w = d.win[nb_index(j, l, d.X, d.Y)];
-                if ((w & 0xFFFFFFFF00000000ull) == stamp) {
-                    const uint32_t wj = static_cast<uint32_t>(w);
-                    keep |= (wj >= j);
-                    if (wj == j) captured |= 1u << l;
-                }
-                else {
-                    keep = 1;  // nobody reached it this step: still liquid for this cell
-                }
+            // Claims on every liquid neighbour, loaded together (masked lanes reload win[j]).
+#pragma unroll
+            for (int l = 0; l < 26; l++) {
+                const uint32_t on = (m >> l) & 1u;
+                const unsigned long long w = d.win[on ? j + gr.off[l] : j];
+                const uint32_t now = (w & 0xFFFFFFFF00000000ull) == stamp;
+                const uint32_t wj = static_cast<uint32_t>(w);
+                // claimed this step: liquid for j only if the winner comes later in index order;
+                // unclaimed: still liquid
+                keep |= on & (now ? static_cast<uint32_t>(wj >= j) : 1u);
+                captured |= (on & now & static_cast<uint32_t>(wj == j)) << l;
             }
             if (!keep) {
                 d.ct[j] = kSolid;
@@ -347,13 +403,13 @@ __global__ __launch_bounds__(kBlock) void k_apply(DomainView d, int p, int layer
                 o = 0;
             }
             const float *g = d.guv + 9 * o;
-            const Coord c = decode(j, d.X, d.Y);
+            const Coord c = decode(j, gr);
             const double cxold = static_cast<float>(d.cx[j]);
             const double cyold = static_cast<float>(d.cy[j]);
             const double czold = static_cast<float>(d.cz[j]);
             for (uint32_t r = captured; r; r &= r - 1) {
                 const int l = __builtin_ctz(r);
-                const uint32_t q = nb_index(j, l, d.X, d.Y);
+                const uint32_t q = j + gr.off[l];
                 const double xp = (c.x + d.xoff + kNbX[l]) + 0.5;
                 const double yp = (c.y + d.yoff + kNbY[l]) + 0.5;
                 const double zp = (c.z + kNbZ[l]) + 0.5;
