@@ -49,6 +49,23 @@ _SIGNATURES = {
     "exaca_profile_steps": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                            ctypes.POINTER(ctypes.c_double), _I64P]),
     "exaca_copy_probe": (ctypes.c_int, [ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
+    "exaca_create_seeded": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_int,
+                                           ctypes.POINTER(ctypes.c_void_p)]),
+    "exaca_prepare": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint64,
+                                     ctypes.POINTER(ctypes.c_void_p)]),
+    "exaca_prepared_info": (ctypes.c_int, [ctypes.c_void_p, _I64P]),
+    "exaca_prepared_free": (ctypes.c_int, [ctypes.c_void_p]),
+    "exaca_ensemble_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
+    "exaca_ensemble_admit": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
+    "exaca_ensemble_release": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "exaca_ensemble_advance": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "exaca_ensemble_poll": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
+    "exaca_ensemble_set_blocks": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "exaca_ensemble_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    "exaca_ensemble_query": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _I64P]),
+    "exaca_ensemble_download": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                               ctypes.c_size_t]),
+    "exaca_ensemble_destroy": (ctypes.c_int, [ctypes.c_void_p]),
 }
 
 _lib = None
@@ -114,13 +131,105 @@ def copy_probe(nbytes: int, reps: int = 20) -> float:
     return ms.value
 
 
+class Prepared:
+    """A type-C deck initialised on the host (any thread, no GPU), ready for an Ensemble."""
+
+    def __init__(self, deck, seed: int = 0, data_dir=None):
+        lib = library()
+        h = ctypes.c_void_p()
+        _ok(lib.exaca_prepare(_b(deck), _b(data_dir) if data_dir else None, seed, ctypes.byref(h)))
+        self._h = h
+        info = (ctypes.c_int64 * 4)()
+        _ok(lib.exaca_prepared_info(h, info))
+        self.cells, self.device_bytes, self.nuclei, self.liquid0 = list(info)
+
+    def close(self):
+        if self._h:
+            _ok(library().exaca_prepared_free(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Ensemble:
+    """Many domains resident on one GPU, stepped together in batched launches."""
+
+    def __init__(self, slots: int, arena_bytes: int):
+        self._lib = library()
+        h = ctypes.c_void_p()
+        _ok(self._lib.exaca_ensemble_create(slots, arena_bytes, ctypes.byref(h)))
+        self._h = h
+        self.slots = slots
+
+    def close(self):
+        if self._h:
+            _ok(self._lib.exaca_ensemble_destroy(self._h))
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def admit(self, prepared: Prepared) -> int | None:
+        """Slot index, or None when no slot or memory is free."""
+        s = ctypes.c_int(-1)
+        st = self._lib.exaca_ensemble_admit(self._h, prepared._h, ctypes.byref(s))
+        if st == 3:
+            return None
+        _ok(st)
+        return s.value
+
+    def release(self, slot: int) -> None:
+        _ok(self._lib.exaca_ensemble_release(self._h, slot))
+
+    def advance(self, steps: int) -> None:
+        _ok(self._lib.exaca_ensemble_advance(self._h, steps))
+
+    def poll(self) -> np.ndarray:
+        out = np.zeros(self.slots, dtype=np.int32)
+        _ok(self._lib.exaca_ensemble_poll(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        return out
+
+    def set_blocks(self, blocks: int) -> None:
+        _ok(self._lib.exaca_ensemble_set_blocks(self._h, blocks))
+
+    def autotune(self, chunks_per_candidate: int = 2) -> int:
+        c = ctypes.c_int(0)
+        _ok(self._lib.exaca_ensemble_autotune(self._h, chunks_per_candidate, ctypes.byref(c)))
+        return c.value
+
+    def query(self, slot: int) -> dict:
+        out = (ctypes.c_int64 * 8)()
+        _ok(self._lib.exaca_ensemble_query(self._h, slot, out))
+        return dict(zip(["final_cycle", "nn", "frontier", "captures", "solidified", "error", "free_bytes",
+                         "steps"], list(out)))
+
+    def field(self, slot: int, kind: int, ncell: int) -> np.ndarray:
+        out = np.empty(ncell, dtype=np.int32)
+        _ok(self._lib.exaca_ensemble_download(self._h, slot, kind, out.ctypes.data, out.nbytes))
+        return out
+
+
 class Solver:
     """One CA domain resident on the GPU."""
 
-    def __init__(self, deck, data_dir=None, verbose: bool = False):
+    def __init__(self, deck, data_dir=None, verbose: bool = False, seed: int = 0):
         self._lib = library()
         h = ctypes.c_void_p()
-        _ok(self._lib.exaca_create(_b(deck), _b(data_dir) if data_dir else None, int(verbose), ctypes.byref(h)))
+        _ok(self._lib.exaca_create_seeded(_b(deck), _b(data_dir) if data_dir else None, seed, int(verbose),
+                                          ctypes.byref(h)))
         self._h = h
         q = self.query()
         self.X, self.Y, self.nz, self.nzA = q["X"], q["Y"], q["nz"], q["nzA"]

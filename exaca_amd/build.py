@@ -33,7 +33,7 @@ HIP_FLAGS = [
 ]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wextra", "-Wno-unused-parameter"]
 
-HIP_SOURCES = ["ca_kernels.hip", "capi.hip"]
+HIP_SOURCES = ["ca_kernels.hip", "capi.hip", "ensemble.hip"]
 CXX_SOURCES = ["deck.cpp", "init.cpp", "output.cpp"]
 
 

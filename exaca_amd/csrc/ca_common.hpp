@@ -48,6 +48,8 @@ struct StepCtl {
     uint32_t count[2];    // frontier list lengths
     int32_t njumps;       // cycle jumps taken so far (spot-melt problems)
     int32_t error;        // nonzero: frontier list overflow or invalid grain
+    int32_t done;         // set by the device when a domain with autostop finishes
+    int32_t final_cycle;  // the check cycle at which it finished
     int32_t jump_from[kMaxJumps];  // cycle J before the jump; cycles J+1 .. T-1 were skipped
     int32_t jump_to[kMaxJumps];    // T, the first cycle executed after the jump
 };
@@ -59,6 +61,7 @@ struct StepStats {
     unsigned long long solidified;      // active cells turned solid
     unsigned long long nucleated;       // nucleation events counted as the reference's nn
     unsigned long long waiting;         // frontier entries still above their liquidus time
+    unsigned long long activated;       // all nucleation events (interior or not)
 };
 
 // Termination-check reduction (src/CAupdate.cpp:738-758 plus the min CritTimeStep of :792-808).
@@ -88,10 +91,14 @@ struct DomainView {
     const int32_t *ntime;
     const float *guv;       // 10000 x 9 unit-vector components (GrainUnitVector)
     const float *seed_cdl;  // 10000 x 26 critical diagonal lengths of cell-centred octahedra
+    const double *reach;    // per orientation: c with critical length^2 >= c |x - centre|^2
     const double *grow;     // growth increment after k executed steps below the liquidus
     int32_t ngrow;          // table length; the last entry holds for every larger k
     StepCtl *ctl;
     StepStats *stats;
+    int32_t autostop;       // end the domain on the device at the first check with no liquid left
+    long long liquid0;      // liquid cells of the layer at the start
+    int32_t *status;        // where the device reports the final cycle of an autostopped domain
 };
 
 }  // namespace exaca

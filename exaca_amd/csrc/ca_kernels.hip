@@ -246,13 +246,12 @@ __device__ __forceinline__ unsigned long long block_sum(unsigned long long v, un
     return t;
 }
 
-// Contiguous slice of the frontier for this workgroup. Workgroups are dispatched round-robin over
-// the 8 XCDs, so consecutive slices are given to workgroups on the same XCD: neighbouring frontier
-// cells (and their neighbour reads and claims) then share one L2.
-__device__ __forceinline__ void frontier_slice(uint32_t count, uint32_t cap, uint32_t &begin, uint32_t &end,
-                                               uint32_t *slice_out = nullptr) {
+// Contiguous slice of the frontier for workgroup b of the nb serving one domain. Workgroups are
+// dispatched round-robin over the 8 XCDs, so consecutive slices are given to workgroups on the
+// same XCD: neighbouring frontier cells (and their neighbour reads and claims) then share one L2.
+__device__ __forceinline__ void frontier_slice(uint32_t count, uint32_t cap, uint32_t b, uint32_t nb,
+                                               uint32_t &begin, uint32_t &end, uint32_t *slice_out = nullptr) {
     const uint32_t n = count < cap ? count : cap;  // an overflowed list is flagged, never over-read
-    const uint32_t nb = gridDim.x, b = blockIdx.x;
     const uint32_t logical = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
     if (slice_out) *slice_out = logical;
     uint32_t chunk = (n + nb - 1) / nb;
@@ -262,16 +261,20 @@ __device__ __forceinline__ void frontier_slice(uint32_t count, uint32_t cap, uin
     end = (b0 + chunk >= n) ? n : static_cast<uint32_t>(b0 + chunk);
 }
 
-__global__ __launch_bounds__(kBlock) void k_nucleate(DomainView d, int p, int layer) {
+// Every body below works on one domain with workgroups b = 0 .. nb-1; the plain launches use the
+// grid x dimension, the ensemble launches add one grid row per domain slot. A finished domain
+// (device-side termination) keeps its state: its bodies return at once.
+__device__ __forceinline__ void nucleate_body(const DomainView &d, int p, int layer, uint32_t b, uint32_t nb) {
     StepCtl *ctl = d.ctl;
+    if (ctl->done) return;
     const int cycle = ctl->cycle[p];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (b == 0 && threadIdx.x == 0) {
         ctl->cycle[1 - p] = cycle + 1;
         ctl->serial[1 - p] = ctl->serial[p] + 1;
         ctl->count[1 - p] = 0;
     }
-    unsigned long long nn = 0;
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < d.nnuclei; i += gridDim.x * kBlock) {
+    unsigned long long nn = 0, act = 0;
+    for (int i = b * kBlock + threadIdx.x; i < d.nnuclei; i += nb * kBlock) {
         const uint32_t q = static_cast<uint32_t>(d.nloc[i]);
         if (!(cycle >= d.ntime[i] && d.ct[q] == kLiquid && d.lid[q] <= layer)) continue;
         const Coord c = decode(q, d.X, d.Y);
@@ -281,6 +284,7 @@ __global__ __launch_bounds__(kBlock) void k_nucleate(DomainView d, int p, int la
         d.cy[q] = (c.y + d.yoff) + 0.5;
         d.cz[q] = c.z + 0.5;
         d.ct[q] = kActive | kSeedBit;
+        act++;
         const uint32_t slot = atomicAdd(&ctl->count[p], 1u);
         if (slot < d.list_cap)
             d.list[p][slot] = q;
@@ -288,19 +292,21 @@ __global__ __launch_bounds__(kBlock) void k_nucleate(DomainView d, int p, int la
             atomicOr(&ctl->error, 1);
     }
     if (nn) atomicAdd(&d.stats->nucleated, nn);
+    if (act) atomicAdd(&d.stats->activated, act);
 }
 
 // Pass A. All per-cell loads (own state, octahedron, the 26 neighbour types) are issued before any
 // of them is used, so a frontier cell costs one memory round trip; a neighbour outside the slab
 // loads the cell's own (active, never liquid) type instead and is masked out.
-__global__ __launch_bounds__(kBlock) void k_claim(DomainView d, int p, int layer) {
+__device__ __forceinline__ void claim_body(const DomainView &d, int p, int layer, uint32_t b, uint32_t nb) {
     __shared__ unsigned long long sh[kBlock / 64];
     const StepCtl *ctl = d.ctl;
+    if (ctl->done) return;
     const int cycle = ctl->cycle[p];
     const unsigned long long stamp = static_cast<unsigned long long>(0xFFFFFFFFu - ctl->serial[p]) << 32;
     const int njumps = ctl->njumps;
     uint32_t begin, end;
-    frontier_slice(ctl->count[p], d.list_cap, begin, end);
+    frontier_slice(ctl->count[p], d.list_cap, b, nb, begin, end);
     const uint32_t *list = d.list[p];
     const Grid gr = grid_of(d);
     unsigned long long front = 0;
@@ -347,8 +353,14 @@ __global__ __launch_bounds__(kBlock) void k_claim(DomainView d, int p, int layer
         else {
             const Faces f = faces_of(d.guv + 9 * o);
             const double xp = (c.x + d.xoff) + 0.5, yp = (c.y + d.yoff) + 0.5, zp = c.z + 0.5;
+            // A neighbour whose distance from the centre bounds its critical length above the
+            // half-diagonal cannot be reached this step; only the others need the exact length.
+            const double reach = d.reach[o];
+            const double dl2 = static_cast<double>(dlj) * static_cast<double>(dlj);
             for (uint32_t m = liquid; m; m &= m - 1) {
                 const int l = __builtin_ctz(m);
+                const double ex = xp + kNbX[l] - cx, ey = yp + kNbY[l] - cy, ez = zp + kNbZ[l] - cz;
+                if (dl2 < reach * (ex * ex + ey * ey + ez * ez)) continue;
                 if (dlj >= critical_length(f, xp, yp, zp, cx, cy, cz, l)) atomicMin(&d.win[j + gr.off[l]], key);
             }
         }
@@ -360,12 +372,13 @@ __global__ __launch_bounds__(kBlock) void k_claim(DomainView d, int p, int layer
 // Pass B. Per frontier entry the outcome is written back over its mask as (keep << 30 | captured
 // neighbours); every workgroup stores the number of next-frontier entries its slice produces so
 // that k_emit can lay the next frontier out in the same order as the current one.
-__global__ __launch_bounds__(kBlock) void k_apply(DomainView d, int p, int layer) {
+__device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t b, uint32_t nb) {
     __shared__ unsigned long long sh[kBlock / 64];
     StepCtl *ctl = d.ctl;
+    if (ctl->done) return;
     const unsigned long long stamp = static_cast<unsigned long long>(0xFFFFFFFFu - ctl->serial[p]) << 32;
     uint32_t begin, end, slice;
-    frontier_slice(ctl->count[p], d.list_cap, begin, end, &slice);
+    frontier_slice(ctl->count[p], d.list_cap, b, nb, begin, end, &slice);
     const uint32_t *list = d.list[p];
     const Grid gr = grid_of(d);
     unsigned long long ncap = 0, nsol = 0, nwait = 0, produced = 0;
@@ -440,14 +453,19 @@ __global__ __launch_bounds__(kBlock) void k_apply(DomainView d, int p, int layer
 // slice each entry's survivor comes first, then its captured neighbours in neighbour order. The
 // front therefore keeps the spatial order of the current list (a column-ordered front stays
 // column-ordered), which keeps the per-cell reads of the next step coalesced.
-__global__ __launch_bounds__(kBlock) void k_emit(DomainView d, int p) {
+// With autostop set, the workgroup of slice 0 also ends the domain at a check step (cycle a
+// multiple of 1000) that finds no liquid left: liquid cells only ever leave the liquid state by
+// capture or nucleation, so the reference's superheated + undercooled count
+// (src/CAupdate.cpp:738-783) is the initial liquid count minus both.
+__device__ __forceinline__ void emit_body(const DomainView &d, int p, uint32_t b, uint32_t nb) {
     __shared__ unsigned long long sh[kBlock / 64];
     __shared__ uint32_t wave_tot[kBlock / 64];
     StepCtl *ctl = d.ctl;
+    if (ctl->done) return;
     uint32_t begin, end, slice;
-    frontier_slice(ctl->count[p], d.list_cap, begin, end, &slice);
+    frontier_slice(ctl->count[p], d.list_cap, b, nb, begin, end, &slice);
     unsigned long long before = 0, all = 0;
-    for (uint32_t s = threadIdx.x; s < gridDim.x; s += kBlock) {
+    for (uint32_t s = threadIdx.x; s < nb; s += kBlock) {
         const uint32_t v = d.slice_out[s];
         all += v;
         if (s < slice) before += v;
@@ -460,12 +478,20 @@ __global__ __launch_bounds__(kBlock) void k_emit(DomainView d, int p) {
         if (slice == 0) {
             ctl->count[1 - p] = static_cast<uint32_t>(all < d.list_cap ? all : d.list_cap);
             if (all > d.list_cap) atomicOr(&ctl->error, 1);
+            const int cycle = ctl->cycle[p];
+            if (d.autostop && cycle % 1000 == 0 &&
+                d.liquid0 - static_cast<long long>(d.stats->captures) - static_cast<long long>(d.stats->activated) == 0) {
+                ctl->done = 1;
+                ctl->final_cycle = cycle;
+                if (d.status) *d.status = cycle;
+            }
         }
     }
     __syncthreads();
     unsigned long long base = base_sh;
     const uint32_t *list = d.list[p];
     uint32_t *next = d.list[1 - p];
+    const Grid gr = grid_of(d);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (uint32_t i0 = begin; i0 < end; i0 += kBlock) {
         const uint32_t i = i0 + threadIdx.x;
@@ -488,11 +514,40 @@ __global__ __launch_bounds__(kBlock) void k_emit(DomainView d, int p) {
         unsigned long long pos = base + wbase + incl - mine;
         if (pos + mine <= d.list_cap) {
             if (keep) next[pos++] = j;
-            for (uint32_t r = captured; r; r &= r - 1) next[pos++] = nb_index(j, __builtin_ctz(r), d.X, d.Y);
+            for (uint32_t r = captured; r; r &= r - 1) next[pos++] = j + gr.off[__builtin_ctz(r)];
         }
         base += round;
         __syncthreads();
     }
+}
+
+// Plain launches: one domain, passed by value.
+__global__ __launch_bounds__(kBlock) void k_nucleate(DomainView d, int p, int layer) {
+    nucleate_body(d, p, layer, blockIdx.x, gridDim.x);
+}
+__global__ __launch_bounds__(kBlock) void k_claim(DomainView d, int p, int layer) {
+    claim_body(d, p, layer, blockIdx.x, gridDim.x);
+}
+__global__ __launch_bounds__(kBlock) void k_apply(DomainView d, int p) { apply_body(d, p, blockIdx.x, gridDim.x); }
+__global__ __launch_bounds__(kBlock) void k_emit(DomainView d, int p) { emit_body(d, p, blockIdx.x, gridDim.x); }
+
+// Ensemble launches: grid row y serves domain slot y of a device-resident table; an empty slot has
+// no control block.
+__global__ __launch_bounds__(kBlock) void k_nucleate_n(const DomainView *v, int p, int layer) {
+    const DomainView &d = v[blockIdx.y];
+    if (d.ctl) nucleate_body(d, p, layer, blockIdx.x, gridDim.x);
+}
+__global__ __launch_bounds__(kBlock) void k_claim_n(const DomainView *v, int p, int layer) {
+    const DomainView &d = v[blockIdx.y];
+    if (d.ctl) claim_body(d, p, layer, blockIdx.x, gridDim.x);
+}
+__global__ __launch_bounds__(kBlock) void k_apply_n(const DomainView *v, int p) {
+    const DomainView &d = v[blockIdx.y];
+    if (d.ctl) apply_body(d, p, blockIdx.x, gridDim.x);
+}
+__global__ __launch_bounds__(kBlock) void k_emit_n(const DomainView *v, int p) {
+    const DomainView &d = v[blockIdx.y];
+    if (d.ctl) emit_body(d, p, blockIdx.x, gridDim.x);
 }
 
 __global__ __launch_bounds__(kBlock) void k_check(DomainView d, int cycle, int layer, CheckResult *out) {
@@ -590,8 +645,16 @@ void launch_claim(const DomainView &d, int parity, int layer, int blocks, hipStr
     k_claim<<<blocks, kBlock, 0, s>>>(d, parity, layer);
 }
 void launch_apply(const DomainView &d, int parity, int layer, int blocks, hipStream_t s) {
-    k_apply<<<blocks, kBlock, 0, s>>>(d, parity, layer);
+    (void)layer;
+    k_apply<<<blocks, kBlock, 0, s>>>(d, parity);
     k_emit<<<blocks, kBlock, 0, s>>>(d, parity);
+}
+void launch_step_n(const DomainView *views, int slots, int parity, int layer, int nuc_blocks, int blocks,
+                   hipStream_t s) {
+    k_nucleate_n<<<dim3(nuc_blocks, slots), kBlock, 0, s>>>(views, parity, layer);
+    k_claim_n<<<dim3(blocks, slots), kBlock, 0, s>>>(views, parity, layer);
+    k_apply_n<<<dim3(blocks, slots), kBlock, 0, s>>>(views, parity);
+    k_emit_n<<<dim3(blocks, slots), kBlock, 0, s>>>(views, parity);
 }
 void launch_check(const DomainView &d, int cycle, int layer, CheckResult *out, hipStream_t s) {
     k_check<<<grid_for(d.ncell, 4096), kBlock, 0, s>>>(d, cycle, layer, out);

@@ -15,6 +15,9 @@ void launch_claim(const DomainView &d, int parity, int layer, int blocks, hipStr
 // CellCapture pass B: winners build new octahedra, losers/finished cells solidify, next frontier
 // (src/CAupdate.cpp:353-725).
 void launch_apply(const DomainView &d, int parity, int layer, int blocks, hipStream_t s);
+// One step of every domain in a device-resident table of `slots` views (grid row = slot).
+void launch_step_n(const DomainView *views, int slots, int parity, int layer, int nuc_blocks, int blocks,
+                   hipStream_t s);
 // IntermediateOutputAndCheck's reduction (src/CAupdate.cpp:738-758, 792-808).
 void launch_check(const DomainView &d, int cycle, int layer, CheckResult *out, hipStream_t s);
 // Octahedra of the initially active cells (src/CAinitialize.cpp:1936-2043).

@@ -20,46 +20,18 @@
 #include "deck.hpp"
 #include "init.hpp"
 #include "output.hpp"
+#include "runtime_util.hpp"
+
+namespace exaca {
+thread_local std::string g_last_error;
+}
 
 namespace {
 
-thread_local std::string g_last_error;
-
-struct DeviceError : std::runtime_error {
-    using std::runtime_error::runtime_error;
-};
-struct StateError : std::runtime_error {
-    using std::runtime_error::runtime_error;
-};
-
-void hip_check(hipError_t e, const char *what) {
-    if (e != hipSuccess) throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));
-}
-#define HIP_OK(x) hip_check((x), #x)
-
-template <typename F>
-int guarded(F &&f) {
-    try {
-        f();
-        return EXACA_OK;
-    }
-    catch (const DeviceError &e) {
-        g_last_error = e.what();
-        return EXACA_ERR_DEVICE;
-    }
-    catch (const StateError &e) {
-        g_last_error = e.what();
-        return EXACA_ERR_STATE;
-    }
-    catch (const std::exception &e) {
-        g_last_error = e.what();
-        return EXACA_ERR_INPUT;
-    }
-    catch (...) {
-        g_last_error = "unknown error";
-        return EXACA_ERR_INPUT;
-    }
-}
+using exaca::DeviceError;
+using exaca::StateError;
+using exaca::guarded;
+using exaca::g_last_error;
 
 constexpr int kGraphSteps = 50;  // even: a graph starts and ends on parity 0
 
@@ -109,7 +81,11 @@ void build_device_state(exaca_state *h) {
     HIP_OK(hipGetDevice(&dev));
     hipDeviceProp_t prop;
     HIP_OK(hipGetDeviceProperties(&prop, dev));
-    h->blocks = ((prop.multiProcessorCount * 8 + 7) / 8) * 8;
+    // Frontier kernels: up to 8 workgroups per CU, fewer for small domains (the frontier never
+    // exceeds the cell count, and a slice is at least one workgroup of cells).
+    const long long cap = ((prop.multiProcessorCount * 8 + 7) / 8) * 8;
+    const long long want = ((hd.ncell / 4096 + 7) / 8) * 8;
+    h->blocks = static_cast<int>(std::max<long long>(8, std::min(cap, want)));
 
     DomainView &d = h->dv;
     d.X = hd.X;
@@ -145,6 +121,10 @@ void build_device_state(exaca_state *h) {
     for (int o = 0; o < kNumOrientations; o++) seed_critical_lengths(&hd.guv[9 * o], &seed[26 * o]);
     float *seed_d = h->alloc<float>(seed.size());
     d.seed_cdl = seed_d;
+    std::vector<double> reach(kNumOrientations);
+    for (int o = 0; o < kNumOrientations; o++) reach[o] = reach_bound(&hd.guv[9 * o]);
+    double *reach_d = h->alloc<double>(reach.size());
+    d.reach = reach_d;
     std::vector<double> grow = growth_table(hd.A, hd.B, hd.C, hd.D, hd.dU, 1 << 25);
     if (grow.size() >= (1u << 25)) throw std::runtime_error("undercooling never saturates within 2^25 steps");
     double *grow_d = h->alloc<double>(grow.size());
@@ -177,6 +157,7 @@ void build_device_state(exaca_state *h) {
     }
     HIP_OK(hipMemcpyAsync(guv, hd.guv.data(), hd.guv.size() * 4, hipMemcpyHostToDevice, s));
     HIP_OK(hipMemcpyAsync(seed_d, seed.data(), seed.size() * 4, hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemcpyAsync(reach_d, reach.data(), reach.size() * 8, hipMemcpyHostToDevice, s));
     HIP_OK(hipMemcpyAsync(grow_d, grow.data(), grow.size() * 8, hipMemcpyHostToDevice, s));
     if (!hd.active.empty()) {
         HIP_OK(hipMemcpyAsync(d.list[0], hd.active.data(), hd.active.size() * 4, hipMemcpyHostToDevice, s));
@@ -324,10 +305,11 @@ void run_layer(exaca_state *h, int layer, int max_cycles, int verbose, int *out_
     if (gpu_ms) *gpu_ms = ms;
 }
 
-exaca_state *create_state(const std::string &input, const std::string &dd, bool verbose) {
+exaca_state *create_state(const std::string &input, const std::string &dd, bool verbose, uint64_t seed = 0) {
     auto t0 = std::chrono::steady_clock::now();
     auto h = std::make_unique<exaca_state>();
     h->deck = exaca::read_deck(input, dd);
+    h->deck.seed = seed;
     h->host = exaca::initialize(h->deck, verbose);
     build_device_state(h.get());
     h->init_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -345,6 +327,14 @@ int exaca_create(const char *input_file, const char *data_dir, int verbose, exac
         if (!input_file || !out) throw std::invalid_argument("null argument");
         *out = nullptr;
         *out = create_state(input_file, data_dir ? data_dir : exaca::data_dir(), verbose != 0);
+    });
+}
+
+int exaca_create_seeded(const char *input_file, const char *data_dir, uint64_t seed, int verbose, exaca_state **out) {
+    return guarded([&] {
+        if (!input_file || !out) throw std::invalid_argument("null argument");
+        *out = nullptr;
+        *out = create_state(input_file, data_dir ? data_dir : exaca::data_dir(), verbose != 0, seed);
     });
 }
 

@@ -1,5 +1,6 @@
 // deck.hpp -- the reference's input-deck format (src/CAinitialize.cpp:24-576).
 #pragma once
+#include <cstdint>
 #include <string>
 
 namespace exaca {
@@ -20,6 +21,10 @@ struct Deck {
     std::string substrate_file;
     bool print_misorientation = false, print_full = false;
     int print_debug = 0;
+    // Random-stream seed of the initial state. 0 reproduces the reference (mt19937_64(0) for the
+    // substrate and the nuclei sites, a default-seeded default_random_engine for the nucleation
+    // undercooling); ensembles give each instance its own.
+    uint64_t seed = 0;
 };
 
 // Directory holding Materials/ and Substrate/ (the reference's ExaCA_DATA_INSTALL /

@@ -88,7 +88,7 @@ void temperature_spots(const Deck &d, HostDomain &h) {
 
 // SubstrateInit_ConstrainedGrowth (src/CAinitialize.cpp:1460-1529), rank 0 of 1.
 void substrate_directional(const Deck &d, HostDomain &h) {
-    std::mt19937_64 gen(0);
+    std::mt19937_64 gen(d.seed);
     std::uniform_real_distribution<double> dis(0.0, 1.0);
     for (int k = 0; k < h.nz; k++)
         for (int i = 0; i < h.X; i++)
@@ -248,7 +248,7 @@ void walls_and_interface(HostDomain &h) {
 // from mt19937_64(0) over cells with melting data in (z, y, x) order, numbered -1, -2, ... in
 // index order; then liquid cells with no non-solid, non-wall neighbour turn solid, in index order.
 void grains(const Deck &d, HostDomain &h) {
-    std::mt19937_64 gen(0);
+    std::mt19937_64 gen(d.seed);
     std::uniform_real_distribution<double> dis(0.0, 1.0);
     const double bulk = d.nmax * d.deltax * d.deltax * d.deltax;
     for (int k = 1; k < h.nz - 1; k++)
@@ -284,7 +284,7 @@ void grains(const Deck &d, HostDomain &h) {
 // NucleiInit (src/CAinitialize.cpp:2213-2248, 2764-2767), rank 0: one normal draw per site in
 // index order from a default-seeded std::default_random_engine.
 void nuclei(const Deck &d, HostDomain &h) {
-    std::default_random_engine generator;
+    std::default_random_engine generator(static_cast<std::default_random_engine::result_type>(d.seed % 2147483646u + 1u));
     std::normal_distribution<double> distribution(d.dTN, d.dTsigma);
     for (long long p = 0; p < h.ncell; p++) {
         if (h.ct[p] != kTemporaryInit) continue;
@@ -334,6 +334,19 @@ void seed_critical_lengths(const float *g, float out[26]) {
         out[n] = static_cast<float>(std::pow(
             std::pow(t * diag[0][0], 2.0) + std::pow(t * diag[0][1], 2.0) + std::pow(t * diag[0][2], 2.0), 0.5));
     }
+}
+
+double reach_bound(const float *g) {
+    double e2 = 0;
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+            double dot = 0;
+            for (int c = 0; c < 3; c++) dot += static_cast<double>(g[3 * a + c]) * g[3 * b + c];
+            const double dev = dot - (a == b ? 1.0 : 0.0);
+            e2 += dev * dev;
+        }
+    const double lo = 1.0 - std::sqrt(e2);
+    return lo > 0 ? 0.999 * lo : 0.0;
 }
 
 std::vector<double> growth_table(double A, double B, double C, double D, float dU, int max_entries) {

@@ -47,6 +47,12 @@ enum exaca_field {
  * uploads it. data_dir holds Materials/ and Substrate/ (NULL: the library's data directory). */
 int exaca_create(const char *input_file, const char *data_dir, int verbose, exaca_state **out);
 
+/* exaca_create with a random-stream seed for the initial state (no reference counterpart: the
+ * reference's seeds are fixed). seed 0 is the reference's own streams; ensemble instances use
+ * their own seeds so that each can be re-run alone on the plain path. */
+int exaca_create_seeded(const char *input_file, const char *data_dir, uint64_t seed, int verbose,
+                        exaca_state **out);
+
 /* Nucleation (src/CAupdate.hpp:15 / src/CAupdate.cpp:67) for time step `cycle` of `layer`.
  * The nucleation count (the reference's nn) accumulates on the device; see exaca_query. */
 int exaca_nucleation(exaca_state *h, int cycle, int layer);
@@ -89,6 +95,41 @@ int exaca_run_program(int id, int np, const char *input_file);
  * is one of CELLTYPE, GRAINID, CRITTIMESTEP, LAYERID, NUCLEI_LOC, NUCLEI_TIME. */
 int exaca_host_state(const char *input_file, const char *data_dir, int field, void *dst, size_t dst_bytes,
                      int64_t dims[8]);
+
+/* ---- Ensembles: many domains resident on one GPU, stepped together (no reference counterpart;
+ * the reference runs one domain per program). Each domain evolves exactly as it does alone on the
+ * plain path: its own cycle, frontier and 1000-step termination check, decided on the device. ---- */
+typedef struct exaca_prepared exaca_prepared;
+typedef struct exaca_ensemble exaca_ensemble;
+
+/* Host-side initialisation of one type-C deck with a random-stream seed (thread-safe, no GPU). */
+int exaca_prepare(const char *input_file, const char *data_dir, uint64_t seed, exaca_prepared **out);
+/* out[0..3]: cells, device bytes needed, nucleation sites, liquid cells at the start. */
+int exaca_prepared_info(const exaca_prepared *pr, int64_t out[4]);
+int exaca_prepared_free(exaca_prepared *pr);
+/* An ensemble of `slots` domain slots sharing one device arena of arena_bytes (the admission
+ * budget). */
+int exaca_ensemble_create(int slots, size_t arena_bytes, exaca_ensemble **out);
+/* Uploads a prepared domain into a free slot; EXACA_ERR_STATE when no slot or memory is free. */
+int exaca_ensemble_admit(exaca_ensemble *e, const exaca_prepared *pr, int *slot);
+/* Frees a slot and its memory. */
+int exaca_ensemble_release(exaca_ensemble *e, int slot);
+/* Enqueues `steps` more steps of every slot (rounded up to 50-step graph replays; asynchronous). */
+int exaca_ensemble_advance(exaca_ensemble *e, int steps);
+/* Waits for the queued steps; final_cycles[slots] receives each slot's final cycle, 0 while it
+ * runs, -1 when empty. */
+int exaca_ensemble_poll(exaca_ensemble *e, int32_t *final_cycles);
+/* Workgroups per slot for the frontier kernels (multiple of 8, 8..256). */
+int exaca_ensemble_set_blocks(exaca_ensemble *e, int blocks);
+/* Online autotuner: runs chunks_per_candidate chunks per workgroup-count candidate on the live
+ * ensemble and keeps the one with the least device time per active-cell update. */
+int exaca_ensemble_autotune(exaca_ensemble *e, int chunks_per_candidate, int *chosen);
+/* out[0..7]: final cycle (0 while running), nucleation events, frontier-size sum, captures,
+ * solidifications, device error flag, free arena bytes, steps enqueued so far. */
+int exaca_ensemble_query(exaca_ensemble *e, int slot, int64_t out[8]);
+/* CELLTYPE or GRAINID of one slot (whole domain, int32). */
+int exaca_ensemble_download(exaca_ensemble *e, int slot, int field, void *dst, size_t dst_bytes);
+int exaca_ensemble_destroy(exaca_ensemble *e);
 
 /* Message of the last failure on this thread. */
 const char *exaca_last_error(void);
