@@ -49,6 +49,8 @@ _SIGNATURES = {
     "exaca_profile_steps": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                            ctypes.POINTER(ctypes.c_double), _I64P]),
     "exaca_copy_probe": (ctypes.c_int, [ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
+    "exaca_profile_kernels": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                             ctypes.POINTER(ctypes.c_double), _I64P]),
     "exaca_create_seeded": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_int,
                                            ctypes.POINTER(ctypes.c_void_p)]),
     "exaca_prepare": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint64,
@@ -301,6 +303,15 @@ class Solver:
         _ok(self._lib.exaca_profile_steps(self._h, layer, nsteps, ms, cnt))
         return {"nucleation_ms": ms[0], "claim_ms": ms[1], "apply_ms": ms[2], "frontier": cnt[0],
                 "captures": cnt[1], "solidified": cnt[2], "launches": cnt[3], "steps": cnt[4]}
+
+    def profile_kernels(self, nsteps: int, layer: int = 0) -> dict:
+        """Runs nsteps more steps with an event pair around every kernel; summed times in ms."""
+        ms = (ctypes.c_double * 6)()
+        cnt = (ctypes.c_int64 * 5)()
+        _ok(self._lib.exaca_profile_kernels(self._h, layer, nsteps, ms, cnt))
+        names = ["k_nucleate", "k_claim", "k_apply", "k_bits_count", "k_bits_scan", "k_bits_write"]
+        return {"ms": dict(zip(names, list(ms))), "frontier": cnt[0], "captures": cnt[1], "solidified": cnt[2],
+                "steps": cnt[4]}
 
     def geometry_probe(self, rows: np.ndarray) -> np.ndarray:
         rows = np.ascontiguousarray(rows, dtype=np.float64).reshape(-1, 7)

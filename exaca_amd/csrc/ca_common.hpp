@@ -96,6 +96,10 @@ struct DomainView {
     int32_t ngrow;          // table length; the last entry holds for every larger k
     StepCtl *ctl;
     StepStats *stats;
+    uint32_t *bits;         // one bit per cell: in the frontier (active)
+    uint32_t *chunk_sum;    // frontier cells per 2048-word chunk of bits, then their exclusive prefix
+    uint32_t nchunks;
+    int32_t index_order;    // rebuild each next frontier from bits, in cell-index order
     int32_t autostop;       // end the domain on the device at the first check with no liquid left
     long long liquid0;      // liquid cells of the layer at the start
     int32_t *status;        // where the device reports the final cycle of an autostopped domain

@@ -284,6 +284,7 @@ __device__ __forceinline__ void nucleate_body(const DomainView &d, int p, int la
         d.cy[q] = (c.y + d.yoff) + 0.5;
         d.cz[q] = c.z + 0.5;
         d.ct[q] = kActive | kSeedBit;
+        if (d.bits) atomicOr(&d.bits[q >> 5], 1u << (q & 31));
         act++;
         const uint32_t slot = atomicAdd(&ctl->count[p], 1u);
         if (slot < d.list_cap)
@@ -369,6 +370,8 @@ __device__ __forceinline__ void claim_body(const DomainView &d, int p, int layer
     if (threadIdx.x == 0 && tot) atomicAdd(&d.stats->frontier, tot);
 }
 
+__device__ __forceinline__ void maybe_stop(const DomainView &d, StepCtl *ctl, int p);
+
 // Pass B. Per frontier entry the outcome is written back over its mask as (keep << 30 | captured
 // neighbours); every workgroup stores the number of next-frontier entries its slice produces so
 // that k_emit can lay the next frontier out in the same order as the current one.
@@ -405,6 +408,7 @@ __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t 
             }
             if (!keep) {
                 d.ct[j] = kSolid;
+                if (d.bits) atomicAnd(&d.bits[j >> 5], ~(1u << (j & 31)));
                 nsol++;
             }
         }
@@ -433,6 +437,7 @@ __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t 
                 d.cy[q] = oc.cy;
                 d.cz[q] = oc.cz;
                 d.ct[q] = kActive;
+                if (d.bits) atomicOr(&d.bits[q >> 5], 1u << (q & 31));
                 ncap++;
             }
         }
@@ -478,13 +483,7 @@ __device__ __forceinline__ void emit_body(const DomainView &d, int p, uint32_t b
         if (slice == 0) {
             ctl->count[1 - p] = static_cast<uint32_t>(all < d.list_cap ? all : d.list_cap);
             if (all > d.list_cap) atomicOr(&ctl->error, 1);
-            const int cycle = ctl->cycle[p];
-            if (d.autostop && cycle % 1000 == 0 &&
-                d.liquid0 - static_cast<long long>(d.stats->captures) - static_cast<long long>(d.stats->activated) == 0) {
-                ctl->done = 1;
-                ctl->final_cycle = cycle;
-                if (d.status) *d.status = cycle;
-            }
+            maybe_stop(d, ctl, p);
         }
     }
     __syncthreads();
@@ -521,6 +520,100 @@ __device__ __forceinline__ void emit_body(const DomainView &d, int p, uint32_t b
     }
 }
 
+// Device-side end of a domain with autostop: at a check step (cycle a multiple of 1000) with no
+// liquid left. Liquid cells only ever leave the liquid state by capture or nucleation, so the
+// reference's superheated + undercooled count (src/CAupdate.cpp:738-783) is the initial liquid
+// count minus both.
+__device__ __forceinline__ void maybe_stop(const DomainView &d, StepCtl *ctl, int p) {
+    const int cycle = ctl->cycle[p];
+    if (d.autostop && cycle % 1000 == 0 &&
+        d.liquid0 - static_cast<long long>(d.stats->captures) - static_cast<long long>(d.stats->activated) == 0) {
+        ctl->done = 1;
+        ctl->final_cycle = cycle;
+        if (d.status) *d.status = cycle;
+    }
+}
+
+// Index-ordered frontier: the next list is every set bit of `bits` in ascending cell index, so a
+// wave's cells are neighbours in the reference's linear order. Three passes over the bitmap:
+// per-chunk counts, one exclusive scan of the chunk counts, and the write.
+constexpr uint32_t kChunkWords = 8 * kBlock;
+
+__device__ __forceinline__ void bits_count_body(const DomainView &d, uint32_t b, uint32_t nb) {
+    __shared__ unsigned long long sh[kBlock / 64];
+    if (d.ctl->done) return;
+    const uint32_t nwords = static_cast<uint32_t>((d.ncell + 31) / 32);
+    for (uint32_t c = b; c < d.nchunks; c += nb) {
+        const uint32_t w0 = c * kChunkWords + threadIdx.x * 8;
+        uint32_t cnt = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            if (w0 + k < nwords) cnt += __builtin_popcount(d.bits[w0 + k]);
+        const unsigned long long t = block_sum(cnt, sh);
+        if (threadIdx.x == 0) d.chunk_sum[c] = static_cast<uint32_t>(t);
+    }
+}
+
+__device__ __forceinline__ void bits_scan_body(const DomainView &d, int p) {
+    __shared__ uint32_t part[kBlock];
+    StepCtl *ctl = d.ctl;
+    if (ctl->done) return;
+    const uint32_t per = (d.nchunks + kBlock - 1) / kBlock;
+    const uint32_t c0 = threadIdx.x * per;
+    uint32_t mine = 0;
+    for (uint32_t c = c0; c < c0 + per && c < d.nchunks; c++) mine += d.chunk_sum[c];
+    part[threadIdx.x] = mine;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long run = 0;
+        for (int t = 0; t < kBlock; t++) {
+            const uint32_t v = part[t];
+            part[t] = static_cast<uint32_t>(run);
+            run += v;
+        }
+        ctl->count[1 - p] = static_cast<uint32_t>(run < d.list_cap ? run : d.list_cap);
+        if (run > d.list_cap) atomicOr(&ctl->error, 1);
+        maybe_stop(d, ctl, p);
+    }
+    __syncthreads();
+    uint32_t run = part[threadIdx.x];
+    for (uint32_t c = c0; c < c0 + per && c < d.nchunks; c++) {
+        const uint32_t v = d.chunk_sum[c];
+        d.chunk_sum[c] = run;
+        run += v;
+    }
+}
+
+__device__ __forceinline__ void bits_write_body(const DomainView &d, int p, uint32_t b, uint32_t nb) {
+    __shared__ uint32_t wave_tot[kBlock / 64];
+    if (d.ctl->done) return;
+    const uint32_t nwords = static_cast<uint32_t>((d.ncell + 31) / 32);
+    uint32_t *next = d.list[1 - p];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (uint32_t c = b; c < d.nchunks; c += nb) {
+        const uint32_t w0 = c * kChunkWords + threadIdx.x * 8;
+        uint32_t w[8], cnt = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            w[k] = (w0 + k < nwords) ? d.bits[w0 + k] : 0u;
+            cnt += __builtin_popcount(w[k]);
+        }
+        uint32_t wtot;
+        const uint32_t incl = wave_inclusive_scan(cnt, wtot);
+        if (lane == 63) wave_tot[wave] = wtot;
+        __syncthreads();
+        uint32_t wbase = 0;
+        for (int q = 0; q < wave; q++) wbase += wave_tot[q];
+        unsigned long long pos = static_cast<unsigned long long>(d.chunk_sum[c]) + wbase + incl - cnt;
+        if (pos + cnt <= d.list_cap) {
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                for (uint32_t r = w[k]; r; r &= r - 1) next[pos++] = (w0 + k) * 32u + __builtin_ctz(r);
+        }
+        __syncthreads();
+    }
+}
+
 // Plain launches: one domain, passed by value.
 __global__ __launch_bounds__(kBlock) void k_nucleate(DomainView d, int p, int layer) {
     nucleate_body(d, p, layer, blockIdx.x, gridDim.x);
@@ -530,6 +623,11 @@ __global__ __launch_bounds__(kBlock) void k_claim(DomainView d, int p, int layer
 }
 __global__ __launch_bounds__(kBlock) void k_apply(DomainView d, int p) { apply_body(d, p, blockIdx.x, gridDim.x); }
 __global__ __launch_bounds__(kBlock) void k_emit(DomainView d, int p) { emit_body(d, p, blockIdx.x, gridDim.x); }
+__global__ __launch_bounds__(kBlock) void k_bits_count(DomainView d) { bits_count_body(d, blockIdx.x, gridDim.x); }
+__global__ __launch_bounds__(kBlock) void k_bits_scan(DomainView d, int p) { bits_scan_body(d, p); }
+__global__ __launch_bounds__(kBlock) void k_bits_write(DomainView d, int p) {
+    bits_write_body(d, p, blockIdx.x, gridDim.x);
+}
 
 // Ensemble launches: grid row y serves domain slot y of a device-resident table; an empty slot has
 // no control block.
@@ -547,7 +645,19 @@ __global__ __launch_bounds__(kBlock) void k_apply_n(const DomainView *v, int p) 
 }
 __global__ __launch_bounds__(kBlock) void k_emit_n(const DomainView *v, int p) {
     const DomainView &d = v[blockIdx.y];
-    if (d.ctl) emit_body(d, p, blockIdx.x, gridDim.x);
+    if (d.ctl && !d.index_order) emit_body(d, p, blockIdx.x, gridDim.x);
+}
+__global__ __launch_bounds__(kBlock) void k_bits_count_n(const DomainView *v) {
+    const DomainView &d = v[blockIdx.y];
+    if (d.ctl && d.index_order) bits_count_body(d, blockIdx.x, gridDim.x);
+}
+__global__ __launch_bounds__(kBlock) void k_bits_scan_n(const DomainView *v, int p) {
+    const DomainView &d = v[blockIdx.y];
+    if (d.ctl && d.index_order) bits_scan_body(d, p);
+}
+__global__ __launch_bounds__(kBlock) void k_bits_write_n(const DomainView *v, int p) {
+    const DomainView &d = v[blockIdx.y];
+    if (d.ctl && d.index_order) bits_write_body(d, p, blockIdx.x, gridDim.x);
 }
 
 __global__ __launch_bounds__(kBlock) void k_check(DomainView d, int cycle, int layer, CheckResult *out) {
@@ -647,7 +757,40 @@ void launch_claim(const DomainView &d, int parity, int layer, int blocks, hipStr
 void launch_apply(const DomainView &d, int parity, int layer, int blocks, hipStream_t s) {
     (void)layer;
     k_apply<<<blocks, kBlock, 0, s>>>(d, parity);
-    k_emit<<<blocks, kBlock, 0, s>>>(d, parity);
+    if (d.index_order) {
+        const int cb = static_cast<int>(d.nchunks < static_cast<uint32_t>(blocks) ? d.nchunks : blocks);
+        k_bits_count<<<cb, kBlock, 0, s>>>(d);
+        k_bits_scan<<<1, kBlock, 0, s>>>(d, parity);
+        k_bits_write<<<cb, kBlock, 0, s>>>(d, parity);
+    }
+    else {
+        k_emit<<<blocks, kBlock, 0, s>>>(d, parity);
+    }
+}
+void launch_step_timed(const DomainView &d, int parity, int layer, int blocks, hipStream_t s, hipEvent_t *ev) {
+    auto rec = [&](int i) { (void)hipEventRecord(ev[i], s); };
+    rec(0);
+    k_nucleate<<<grid_for(d.nnuclei, 1024), kBlock, 0, s>>>(d, parity, layer);
+    rec(1);
+    rec(2);
+    k_claim<<<blocks, kBlock, 0, s>>>(d, parity, layer);
+    rec(3);
+    rec(4);
+    k_apply<<<blocks, kBlock, 0, s>>>(d, parity);
+    rec(5);
+    const int cb = static_cast<int>(d.nchunks < static_cast<uint32_t>(blocks) ? d.nchunks : blocks);
+    rec(6);
+    if (d.index_order)
+        k_bits_count<<<cb, kBlock, 0, s>>>(d);
+    else
+        k_emit<<<blocks, kBlock, 0, s>>>(d, parity);
+    rec(7);
+    rec(8);
+    if (d.index_order) k_bits_scan<<<1, kBlock, 0, s>>>(d, parity);
+    rec(9);
+    rec(10);
+    if (d.index_order) k_bits_write<<<cb, kBlock, 0, s>>>(d, parity);
+    rec(11);
 }
 void launch_step_n(const DomainView *views, int slots, int parity, int layer, int nuc_blocks, int blocks,
                    hipStream_t s) {
@@ -655,6 +798,9 @@ void launch_step_n(const DomainView *views, int slots, int parity, int layer, in
     k_claim_n<<<dim3(blocks, slots), kBlock, 0, s>>>(views, parity, layer);
     k_apply_n<<<dim3(blocks, slots), kBlock, 0, s>>>(views, parity);
     k_emit_n<<<dim3(blocks, slots), kBlock, 0, s>>>(views, parity);
+    k_bits_count_n<<<dim3(blocks, slots), kBlock, 0, s>>>(views);
+    k_bits_scan_n<<<dim3(1, slots), kBlock, 0, s>>>(views, parity);
+    k_bits_write_n<<<dim3(blocks, slots), kBlock, 0, s>>>(views, parity);
 }
 void launch_check(const DomainView &d, int cycle, int layer, CheckResult *out, hipStream_t s) {
     k_check<<<grid_for(d.ncell, 4096), kBlock, 0, s>>>(d, cycle, layer, out);

@@ -22,6 +22,11 @@ void launch_step_n(const DomainView *views, int slots, int parity, int layer, in
 void launch_check(const DomainView &d, int cycle, int layer, CheckResult *out, hipStream_t s);
 // Octahedra of the initially active cells (src/CAinitialize.cpp:1936-2043).
 void launch_seed_init(const DomainView &d, const uint32_t *cells, uint32_t n, hipStream_t s);
+// One step with an event pair around each kernel: nucleation, claim, apply, and the frontier
+// rebuild (count, scan, write for the index order; emit alone for the capture order, the two
+// unused pairs then time nothing). ev holds 2 * kTimedKernels events.
+constexpr int kTimedKernels = 6;
+void launch_step_timed(const DomainView &d, int parity, int layer, int blocks, hipStream_t s, hipEvent_t *ev);
 // Plain copy of n 16-byte words (HBM calibration).
 void launch_copy(const uint4 *src, uint4 *dst, size_t n, hipStream_t s);
 // Device-side self test of the capture geometry against host values: one octahedron per input row.

@@ -26,8 +26,24 @@ namespace exaca {
 thread_local std::string g_last_error;
 }
 
+namespace exaca {
+// Frontier order of new domains: cell-index order rebuilt from the active bitmap each step
+// (EXACA_FRONTIER_ORDER=index, default) or the capture order kept by k_emit (=capture).
+int frontier_index_order() {
+    const char *v = std::getenv("EXACA_FRONTIER_ORDER");
+    return (v && std::string(v) == "capture") ? 0 : 1;
+}
+std::vector<uint32_t> active_bits(const HostDomain &hd) {
+    std::vector<uint32_t> b((static_cast<size_t>(hd.ncell) + 31) / 32, 0u);
+    for (uint32_t q : hd.active) b[q >> 5] |= 1u << (q & 31);
+    return b;
+}
+}  // namespace exaca
+
 namespace {
 
+using exaca::active_bits;
+using exaca::frontier_index_order;
 using exaca::DeviceError;
 using exaca::StateError;
 using exaca::guarded;
@@ -110,6 +126,11 @@ void build_device_state(exaca_state *h) {
     d.list[1] = h->alloc<uint32_t>(n);
     d.mask = h->alloc<uint32_t>(n);
     d.slice_out = h->alloc<uint32_t>(h->blocks);
+    const size_t nwords = (n + 31) / 32;
+    d.nchunks = static_cast<uint32_t>((nwords + 2047) / 2048);
+    d.bits = h->alloc<uint32_t>(nwords);
+    d.chunk_sum = h->alloc<uint32_t>(d.nchunks);
+    d.index_order = frontier_index_order();
     d.nnuclei = static_cast<int32_t>(hd.nloc.size());
     int32_t *nloc = h->alloc<int32_t>(hd.nloc.size());
     int32_t *ntime = h->alloc<int32_t>(hd.ntime.size());
@@ -159,6 +180,8 @@ void build_device_state(exaca_state *h) {
     HIP_OK(hipMemcpyAsync(seed_d, seed.data(), seed.size() * 4, hipMemcpyHostToDevice, s));
     HIP_OK(hipMemcpyAsync(reach_d, reach.data(), reach.size() * 8, hipMemcpyHostToDevice, s));
     HIP_OK(hipMemcpyAsync(grow_d, grow.data(), grow.size() * 8, hipMemcpyHostToDevice, s));
+    const std::vector<uint32_t> bits = active_bits(hd);
+    HIP_OK(hipMemcpyAsync(d.bits, bits.data(), bits.size() * 4, hipMemcpyHostToDevice, s));
     if (!hd.active.empty()) {
         HIP_OK(hipMemcpyAsync(d.list[0], hd.active.data(), hd.active.size() * 4, hipMemcpyHostToDevice, s));
         launch_seed_init(d, d.list[0], static_cast<uint32_t>(hd.active.size()), s);
@@ -314,6 +337,54 @@ exaca_state *create_state(const std::string &input, const std::string &dd, bool 
     build_device_state(h.get());
     h->init_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return h.release();
+}
+
+// Steps one at a time with HIP events around every kernel (no graphs): per-kernel device time of
+// nucleation, claim, apply and the frontier rebuild (emit, or bitmap count / scan / write).
+void profile(exaca_state *h, int layer, int nsteps, double *ms, int64_t counts[5], int64_t *launches) {
+    using exaca::kTimedKernels;
+    exaca::StepStats before, after;
+    HIP_OK(hipStreamSynchronize(h->stream));
+    HIP_OK(hipMemcpy(&before, h->stats, sizeof before, hipMemcpyDeviceToHost));
+    const int per_batch = 128;
+    std::vector<hipEvent_t> ev(2 * kTimedKernels * per_batch);
+    for (auto &e : ev) HIP_OK(hipEventCreate(&e));
+    double acc[kTimedKernels] = {0, 0, 0, 0, 0, 0};
+    int done = 0, xswitch = 0;
+    while (done < nsteps && !xswitch) {
+        const int batch = std::min(per_batch, nsteps - done);
+        int k = 0;
+        for (; k < batch; k++) {
+            exaca::launch_step_timed(h->dv, h->parity, layer, h->blocks, h->stream, &ev[2 * kTimedKernels * k]);
+            h->parity ^= 1;
+            h->cycle += 1;
+            if (h->cycle % 1000 == 0) {
+                k++;
+                break;
+            }
+        }
+        HIP_OK(hipStreamSynchronize(h->stream));
+        for (int q = 0; q < k; q++)
+            for (int r = 0; r < kTimedKernels; r++) {
+                float t = 0;
+                HIP_OK(hipEventElapsedTime(&t, ev[2 * kTimedKernels * q + 2 * r], ev[2 * kTimedKernels * q + 2 * r + 1]));
+                acc[r] += t;
+            }
+        done += k;
+        if (h->cycle % 1000 == 0) {
+            int c = h->cycle;
+            device_check(h, &c, layer, nullptr, &xswitch, 0);
+        }
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    HIP_OK(hipMemcpy(&after, h->stats, sizeof after, hipMemcpyDeviceToHost));
+    for (int r = 0; r < kTimedKernels; r++) ms[r] = acc[r];
+    counts[0] = static_cast<int64_t>(after.frontier - before.frontier);
+    counts[1] = static_cast<int64_t>(after.captures - before.captures);
+    counts[2] = static_cast<int64_t>(after.solidified - before.solidified);
+    counts[3] = done;
+    counts[4] = done;
+    *launches = done;
 }
 
 }  // namespace
@@ -473,56 +544,22 @@ int exaca_profile_steps(exaca_state *h, int layer, int nsteps, double ms[3], int
     return guarded([&] {
         if (!h || !ms || !counts || nsteps < 0) throw std::invalid_argument("bad argument");
         if (h->pending >= 0) throw StateError("profile_steps called between nucleation and capture");
-        exaca::StepStats before, after;
-        HIP_OK(hipStreamSynchronize(h->stream));
-        HIP_OK(hipMemcpy(&before, h->stats, sizeof before, hipMemcpyDeviceToHost));
-        const int per_batch = 256;
-        std::vector<hipEvent_t> ev(6 * per_batch);
-        for (auto &e : ev) HIP_OK(hipEventCreate(&e));
-        double acc[3] = {0, 0, 0};
-        int done = 0, xswitch = 0;
-        while (done < nsteps && !xswitch) {
-            const int batch = std::min(per_batch, nsteps - done);
-            int k = 0;
-            for (; k < batch; k++) {
-                hipEvent_t *e = &ev[6 * k];
-                HIP_OK(hipEventRecord(e[0], h->stream));
-                step_nucleation(h, layer);
-                HIP_OK(hipEventRecord(e[1], h->stream));
-                HIP_OK(hipEventRecord(e[2], h->stream));
-                exaca::launch_claim(h->dv, h->parity, layer, h->blocks, h->stream);
-                HIP_OK(hipEventRecord(e[3], h->stream));
-                HIP_OK(hipEventRecord(e[4], h->stream));
-                exaca::launch_apply(h->dv, h->parity, layer, h->blocks, h->stream);
-                HIP_OK(hipEventRecord(e[5], h->stream));
-                h->parity ^= 1;
-                h->cycle += 1;
-                if (h->cycle % 1000 == 0) {
-                    k++;
-                    break;
-                }
-            }
-            HIP_OK(hipStreamSynchronize(h->stream));
-            for (int q = 0; q < k; q++)
-                for (int r = 0; r < 3; r++) {
-                    float t = 0;
-                    HIP_OK(hipEventElapsedTime(&t, ev[6 * q + 2 * r], ev[6 * q + 2 * r + 1]));
-                    acc[r] += t;
-                }
-            done += k;
-            if (h->cycle % 1000 == 0) {
-                int c = h->cycle;
-                device_check(h, &c, layer, nullptr, &xswitch, 0);
-            }
-        }
-        for (auto &e : ev) (void)hipEventDestroy(e);
-        HIP_OK(hipMemcpy(&after, h->stats, sizeof after, hipMemcpyDeviceToHost));
-        for (int r = 0; r < 3; r++) ms[r] = acc[r];
-        counts[0] = static_cast<int64_t>(after.frontier - before.frontier);
-        counts[1] = static_cast<int64_t>(after.captures - before.captures);
-        counts[2] = static_cast<int64_t>(after.solidified - before.solidified);
-        counts[3] = done;
-        counts[4] = done;
+        double all[exaca::kTimedKernels];
+        int64_t launches = 0;
+        profile(h, layer, nsteps, all, counts, &launches);
+        ms[0] = all[0];
+        ms[1] = all[1];
+        ms[2] = 0;
+        for (int k = 2; k < exaca::kTimedKernels; k++) ms[2] += all[k];
+    });
+}
+
+int exaca_profile_kernels(exaca_state *h, int layer, int nsteps, double ms[6], int64_t counts[5]) {
+    return guarded([&] {
+        if (!h || !ms || !counts || nsteps < 0) throw std::invalid_argument("bad argument");
+        if (h->pending >= 0) throw StateError("profile_kernels called between nucleation and capture");
+        int64_t launches = 0;
+        profile(h, layer, nsteps, ms, counts, &launches);
     });
 }
 

@@ -23,6 +23,11 @@
 #include "init.hpp"
 #include "runtime_util.hpp"
 
+namespace exaca {
+int frontier_index_order();
+std::vector<uint32_t> active_bits(const HostDomain &hd);
+}  // namespace exaca
+
 namespace {
 
 using exaca::DeviceError;
@@ -81,6 +86,8 @@ size_t domain_bytes(const exaca::HostDomain &h, size_t ngrow) {
     b += 3 * up(4 * n);                                                                  // lists + mask
     b += up(4 * kMaxSlices) + 2 * up(4 * std::max<size_t>(h.nloc.size(), 1)) + up(8 * ngrow);
     b += up(sizeof(exaca::StepCtl)) + up(sizeof(exaca::StepStats));
+    const size_t nwords = (n + 31) / 32;
+    b += up(4 * nwords) + up(4 * ((nwords + 2047) / 2048));
     return b;
 }
 
@@ -211,6 +218,11 @@ void admit(exaca_ensemble *e, const exaca_prepared *pr, int *slot_out) {
     int32_t *nloc = reinterpret_cast<int32_t *>(carve(4 * std::max<size_t>(hd.nloc.size(), 1)));
     int32_t *ntime = reinterpret_cast<int32_t *>(carve(4 * std::max<size_t>(hd.nloc.size(), 1)));
     double *grow = reinterpret_cast<double *>(carve(8 * pr->grow.size()));
+    const size_t nwords = (n + 31) / 32;
+    d.nchunks = static_cast<uint32_t>((nwords + 2047) / 2048);
+    d.bits = reinterpret_cast<uint32_t *>(carve(4 * nwords));
+    d.chunk_sum = reinterpret_cast<uint32_t *>(carve(4 * d.nchunks));
+    d.index_order = exaca::frontier_index_order();
     d.ctl = reinterpret_cast<StepCtl *>(carve(sizeof(StepCtl)));
     d.stats = reinterpret_cast<StepStats *>(carve(sizeof(StepStats)));
     d.nnuclei = static_cast<int32_t>(hd.nloc.size());
@@ -239,6 +251,8 @@ void admit(exaca_ensemble *e, const exaca_prepared *pr, int *slot_out) {
         HIP_OK(hipMemcpyAsync(ntime, hd.ntime.data(), 4 * hd.ntime.size(), hipMemcpyHostToDevice, s));
     }
     HIP_OK(hipMemcpyAsync(grow, pr->grow.data(), 8 * pr->grow.size(), hipMemcpyHostToDevice, s));
+    const std::vector<uint32_t> bits = exaca::active_bits(hd);
+    HIP_OK(hipMemcpyAsync(d.bits, bits.data(), 4 * bits.size(), hipMemcpyHostToDevice, s));
     if (!hd.active.empty()) {
         HIP_OK(hipMemcpyAsync(d.list[0], hd.active.data(), 4 * hd.active.size(), hipMemcpyHostToDevice, s));
         launch_seed_init(d, d.list[0], static_cast<uint32_t>(hd.active.size()), s);
