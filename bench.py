@@ -8,15 +8,21 @@ exactly --steps steps run from the device-resident state (HIP graphs, terminatio
 timed steps of the frontier size (cells past their liquidus time, the reference's numSteer)
 divided by the device time of those steps.
 
-roofline: the dominant kernel pair is CellCapture (k_claim + k_apply). Its algorithmic bytes are
-137 B per active-cell update + 125 B per capture + 1 B per solidification (the reference layout's
-bytes: own state, the 26 critical lengths and the new cell's octahedron), divided by the kernels'
-event-timed durations over --profile-steps further steps.
+roofline: the dominant work is CellCapture (k_claim, k_apply, k_settle and the frontier rebuild).
+Its algorithmic bytes are 137 B per active-cell update + 125 B per capture + 1 B per
+solidification (the reference layout's bytes: own state, the 26 critical lengths and the new cell's
+octahedron), divided by the kernels' event-timed durations over --profile-steps further steps.
 
 cpu_baseline: the oracle (the serial C restatement of the reference's algorithm) on a bounded
 sample of the same workload: the first --cpu-cycles steps of the same deck cut to its bottom
 --cpu-z planes (the random streams are drawn plane by plane, so the substrate, nuclei and their
 times in those planes are the 800^3 deck's own); its CellCapture time as in the reference's log.
+
+ensemble: --ensemble process-map requests (directional solidification, cube sides log-uniform in
+[32, 256] cells, G, R and nucleation density log-uniform, one seed each, Poisson arrivals at
+50 req/s) solved three ways on the same GPU: all at once in the batched ensemble, served as they
+arrive (continuous batching under a 200 GiB budget), and one after another on the plain path.
+solves/s, latency percentiles and whether every result is identical across the three runs.
 """
 from __future__ import annotations
 
@@ -60,6 +66,37 @@ def cpu_baseline(deck: Path, z: int, cycles: int) -> dict:
     }
 
 
+def ensemble_leg(n: int) -> dict:
+    from exaca_amd import decks, serve
+
+    with tempfile.TemporaryDirectory() as d:
+        reqs = decks.process_map_requests(n, d, rate=50.0)
+        cells = sum((r.nx + 2) * (r.ny + 2) * (r.nz + 2) for r in reqs)
+        batched = serve.run_served(reqs, realtime=False)
+        print(f"[bench] batched ensemble: {n} solves in {batched['makespan_s']:.2f} s", file=sys.stderr, flush=True)
+        served = serve.run_served(reqs, realtime=True)
+        print(f"[bench] served: p50 {served['p50_s']:.3f} s p99 {served['p99_s']:.3f} s", file=sys.stderr, flush=True)
+        seq = serve.run_sequential(reqs)
+        print(f"[bench] sequential plain path: {seq['wall_s']:.2f} s", file=sys.stderr, flush=True)
+    identical = all(seq["digests"][k] == batched["digests"][k] == served["digests"][k] for k in seq["digests"])
+    return {
+        "requests": n,
+        "cells": cells,
+        "batched_solves_per_s": batched["solves_per_s"],
+        "served_solves_per_s": served["solves_per_s"],
+        "served_p50_s": served["p50_s"],
+        "served_p99_s": served["p99_s"],
+        "sequential_solves_per_s": seq["solves_per_s"],
+        "sequential_p50_s": seq["p50_s"],
+        "sequential_p99_s": seq["p99_s"],
+        "speedup_served_vs_sequential": served["solves_per_s"] / seq["solves_per_s"],
+        "speedup_batched_vs_sequential": batched["solves_per_s"] / seq["solves_per_s"],
+        "blocks_per_slot": batched["blocks_per_slot"],
+        "identical_outputs": identical,
+        "arrivals": "Poisson 50 req/s (served, sequential); all at t = 0 (batched)",
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +107,7 @@ def main() -> int:
     ap.add_argument("--cpu-z", type=int, default=40)
     ap.add_argument("--cpu-cycles", type=int, default=150)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ensemble", type=int, default=64, help="process-map requests (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs one GPU: the whole CA domain lives on a single MI355X (asked for {args.gpus})",
@@ -108,9 +146,9 @@ def main() -> int:
     cap_ms = sum(v for k, v in kms.items() if k != "k_nucleate")
     bytes_alg = 137 * prof["frontier"] + 125 * prof["captures"] + 1 * prof["solidified"]
     achieved = bytes_alg / (cap_ms / 1e3) / 1e9 if cap_ms > 0 else 0.0
-    # k_claim alone with the bytes this layout moves per frontier cell: list 4, CritTimeStep 4,
-    # LayerID 1, DiagonalLength 4 + 4, CellType 1, GrainID 4, double centre 24, mask 4
-    claim_bytes = 50 * prof["frontier"]
+    # k_claim alone with the bytes this layout moves per gated frontier cell: list 4,
+    # CritTimeStep 4, LayerID 1, DiagonalLength 4 + 4, capture threshold 4, liquid mask 8
+    claim_bytes = 29 * prof["frontier"]
     claim_gbps = claim_bytes / (kms["k_claim"] / 1e3) / 1e9 if kms["k_claim"] > 0 else 0.0
     sol.close()
 
@@ -138,7 +176,8 @@ def main() -> int:
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "CellCapture = k_claim + k_apply + frontier rebuild (k_bits_count, k_bits_scan, k_bits_write)",
+            "kernel": "CellCapture = k_claim + k_apply + k_settle + frontier rebuild (k_bits_count, k_bits_scan, "
+                      "k_bits_write)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
@@ -154,13 +193,15 @@ def main() -> int:
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": claim_gbps / HBM_PEAK_GBPS,
-            "bytes_per_update": 50,
+            "bytes_per_update": 29,
         },
         "kernel_avg_us": {k: 1e3 * v / nsteps for k, v in kms.items()},
         "copy_GBps": copy_gbps,
         "init_s": init_s,
         "wall_s": wall,
     }
+    if args.ensemble > 0:
+        line["ensemble"] = ensemble_leg(args.ensemble)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(deck, args.cpu_z, args.cpu_cycles)
     print(json.dumps(line), flush=True)

@@ -7,8 +7,13 @@
 //   cts  int32   CritTimeStep (liquidus crossing step)
 //   lid  int8    LayerID
 //   dl   float   octahedron half-diagonal (DiagonalLength)
+//   thr  float   lower bound on the critical lengths of the still-liquid neighbours (0: unknown)
 //   cx/cy/cz double  octahedron centre as computed at capture (DOCenter is its float rounding)
-//   win  uint64  capture claim key of the current step (generation stamp << 32 | capturer)
+//   lm   uint64  liquid neighbours of an active cell: bits 0-25 now, bits 32-57 as they were at
+//                this step's claim; bit 31 set: not yet known (a new cell, or never active)
+//   win  uint32  lowest index of a cell that claimed this liquid cell in the current step
+//                (0xFFFFFFFF: unclaimed). A claimed cell is captured in the same step and never
+//                turns liquid again, so a claim word is read in the step that wrote it only.
 // The reference's 26 critical diagonal lengths per cell (104 B) are not stored: they are
 // recomputed bit-exactly from the double centre (captured cells) or read from a per-orientation
 // table (cell-centred octahedra: nuclei and the initial substrate).
@@ -31,7 +36,8 @@ enum CellType : uint8_t {
 constexpr uint8_t kSeedBit = 0x10;
 constexpr uint8_t kTypeMask = 0x0F;
 constexpr int kNumOrientations = 10000;
-constexpr uint32_t kWaitMask = 0x80000000u;  // frontier entry not yet past its liquidus time
+constexpr unsigned long long kLmUnknown = 1ull << 31;
+constexpr unsigned long long kLmNow = (1ull << 26) - 1;
 constexpr int kMaxJumps = 4096;
 
 // The 26 neighbour offsets in the reference order: 0-8 at dy = -1, 9-16 in the y plane, 17-25 at
@@ -39,13 +45,42 @@ constexpr int kMaxJumps = 4096;
 constexpr int kNbX[26] = {0, 1, -1, 0, 0, -1, 1, -1, 1, 0, 0, 1, -1, 1, -1, 1, -1, 0, 1, -1, 0, 0, 1, -1, 1, -1};
 constexpr int kNbY[26] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 1, 1, 1, 1};
 constexpr int kNbZ[26] = {0, 0, 0, 1, -1, 1, 1, -1, -1, 1, -1, 1, 1, -1, -1, 0, 0, 0, 0, 0, 1, -1, 1, 1, -1, -1};
+// The same offsets packed two bits apiece (offset + 1 at bits 2l, 2l+1), so that device code
+// decodes a per-lane neighbour number with shifts; an indexed table would be placed in scratch.
+constexpr unsigned long long pack_offsets(const int (&t)[26]) {
+    unsigned long long v = 0;
+    for (int l = 0; l < 26; l++) v |= static_cast<unsigned long long>(t[l] + 1) << (2 * l);
+    return v;
+}
+constexpr unsigned long long kPackX = pack_offsets(kNbX);
+constexpr unsigned long long kPackY = pack_offsets(kNbY);
+constexpr unsigned long long kPackZ = pack_offsets(kNbZ);
+constexpr unsigned kClaimNone = 0xFFFFFFFFu;
+// Neighbour number of the opposite offset: cell q is neighbour l of r iff r is neighbour
+// kOpposite[l] of q.
+constexpr int opposite_of(int l) {
+    for (int k = 0; k < 26; k++)
+        if (kNbX[k] == -kNbX[l] && kNbY[k] == -kNbY[l] && kNbZ[k] == -kNbZ[l]) return k;
+    return -1;
+}
+struct OppositeTable {
+    int v[26];
+};
+constexpr OppositeTable make_opposites() {
+    OppositeTable t{};
+    for (int l = 0; l < 26; l++) t.v[l] = opposite_of(l);
+    return t;
+}
+constexpr OppositeTable kOpposite = make_opposites();
 
 // Device-resident step control block, double-buffered by step parity so that the kernels of a
 // step read slot p while the first kernel prepares slot 1-p for the next step.
 struct StepCtl {
     int32_t cycle[2];     // time step number seen by the kernels of a step of parity p
-    uint32_t serial[2];   // executed-step counter (claim generation stamp)
     uint32_t count[2];    // frontier list lengths
+    uint32_t rlo[2];      // cell-index range [rlo, rhi] that holds every active cell of the step
+    uint32_t rhi[2];      //   (rlo > rhi: none); the frontier rebuild scans only this range
+    uint32_t ncand;       // cells whose last liquid neighbour was captured in the current step
     int32_t njumps;       // cycle jumps taken so far (spot-melt problems)
     int32_t error;        // nonzero: frontier list overflow or invalid grain
     int32_t done;         // set by the device when a domain with autostop finishes
@@ -80,11 +115,15 @@ struct DomainView {
     int32_t *cts;
     int8_t *lid;
     float *dl;
+    float *thr;
     double *cx, *cy, *cz;
-    unsigned long long *win;
-    uint32_t *list[2];
-    uint32_t *mask;
-    uint32_t *slice_out;    // next-frontier entries produced by each workgroup slice
+    unsigned long long *lm;
+    uint32_t *win;
+    uint32_t *list[2];      // frontier of a step of parity p (list[1-p] also holds the step's
+                            // solidification candidates until the next frontier is written)
+    uint32_t *claim_cell;   // cells with claims in the current step, stored in their own frontier
+    uint32_t *claim_bits;   //   slice, and the neighbours each claimed
+    uint32_t *slice_claims; // entries of each workgroup's slice of the claim lists
     uint32_t list_cap;
     int32_t nnuclei;
     const int32_t *nloc;
@@ -99,7 +138,6 @@ struct DomainView {
     uint32_t *bits;         // one bit per cell: in the frontier (active)
     uint32_t *chunk_sum;    // frontier cells per 2048-word chunk of bits, then their exclusive prefix
     uint32_t nchunks;
-    int32_t index_order;    // rebuild each next frontier from bits, in cell-index order
     int32_t autostop;       // end the domain on the device at the first check with no liquid left
     long long liquid0;      // liquid cells of the layer at the start
     int32_t *status;        // where the device reports the final cycle of an autostopped domain

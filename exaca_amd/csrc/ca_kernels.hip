@@ -1,13 +1,14 @@
 // ca_kernels.hip -- the CA time step as memory-bound CDNA4 kernels over a compacted frontier.
 //
-// One step (reference: src/runCA.cpp:291-311) is three launches on one stream:
+// One step (reference: src/runCA.cpp:291-311) is seven launches on one stream:
 //   k_nucleate  activates the pre-sampled nuclei whose time has come (src/CAupdate.cpp:67-200)
-//   k_claim     for every frontier cell past its liquidus time: growth of its octahedron, scan of
-//               the 26 neighbours, and a claim on every liquid neighbour it reaches, written as
-//               atomicMin of (generation << 32 | capturer index) into win[]
-//   k_apply     the winner of each claimed cell builds the new decentred octahedron; a frontier
-//               cell whose liquid neighbours are all taken by lower-indexed cells turns solid;
-//               survivors and new cells are appended to the next frontier list
+//   k_claim     for every frontier cell past its liquidus time: growth of its octahedron and a
+//               claim on every liquid neighbour it reaches, written as atomicMin of the
+//               capturer's index into win[]; a cell with no liquid neighbour turns solid
+//   k_apply     the winner of each claimed cell builds the new decentred octahedron and takes the
+//               cell out of its neighbours' liquid masks
+//   k_settle    a cell whose liquid neighbours were all taken by lower-indexed cells turns solid
+//   k_bits_*    the next frontier: the active cells in index order, from a bitmap
 // Claim-then-apply reproduces the reference's Serial backend exactly: there pass 2 visits active
 // cells in ascending index order and every capture is visible at once, so a liquid cell goes to
 // the lowest-indexed cell that reaches it, and cell j still "sees" a liquid neighbour only if no
@@ -63,10 +64,19 @@ struct Divisor {
         return (t + ((n - t) >> sh1)) >> sh2;
     }
 };
+// Offsets of neighbour l (a per-lane value) decoded from the packed constants.
+__device__ __forceinline__ int nb_dx(int l) { return static_cast<int>((kPackX >> (2 * l)) & 3u) - 1; }
+__device__ __forceinline__ int nb_dy(int l) { return static_cast<int>((kPackY >> (2 * l)) & 3u) - 1; }
+__device__ __forceinline__ int nb_dz(int l) { return static_cast<int>((kPackZ >> (2 * l)) & 3u) - 1; }
+
 struct Grid {
     Divisor xy, y;
     int X, Y, nzA;
-    int off[26];  // linear offsets of the 26 neighbours
+    int off[26];  // linear offsets of the 26 neighbours, for loops unrolled over l
+    // linear offset of neighbour l for a per-lane l
+    __device__ __forceinline__ int offset(int l) const {
+        return nb_dz(l) * static_cast<int>(xy.d) + nb_dx(l) * Y + nb_dy(l);
+    }
 };
 __device__ __forceinline__ Grid grid_of(const DomainView &d) {
     Grid g;
@@ -97,10 +107,6 @@ __device__ __forceinline__ Coord decode(uint32_t p, int X, int Y) {
     c.x = static_cast<int>(rem / static_cast<uint32_t>(Y));
     c.y = static_cast<int>(rem - static_cast<uint32_t>(c.x) * static_cast<uint32_t>(Y));
     return c;
-}
-__device__ __forceinline__ uint32_t nb_index(uint32_t p, int l, int X, int Y) {
-    return static_cast<uint32_t>(static_cast<long long>(p) + (static_cast<long long>(kNbZ[l]) * X + kNbX[l]) * Y +
-                                 kNbY[l]);
 }
 // Neighbours inside the rank's slab and below the active-domain top; the reference's edge lists
 // (src/CAupdate.cpp:276-318) and its MyNeighborZ < nzActive test (:328). z >= 0 always holds for
@@ -142,7 +148,7 @@ __device__ __forceinline__ Faces faces_of(const float *g) {
 // the reference stores it.
 __device__ __forceinline__ float critical_length(const Faces &f, double xp, double yp, double zp, double cx,
                                                  double cy, double cz, int l) {
-    const double ex = xp + kNbX[l] - cx, ey = yp + kNbY[l] - cy, ez = zp + kNbZ[l] - cz;
+    const double ex = xp + nb_dx(l) - cx, ey = yp + nb_dy(l) - cy, ez = zp + nb_dz(l) - cz;
     const double d0 = ex * f.fx[0] + ey * f.fy[0] + ez * f.fz[0];
     const double d1 = ex * f.fx[1] + ey * f.fy[1] + ez * f.fz[1];
     const double d2 = ex * f.fx[2] + ey * f.fy[2] + ez * f.fz[2];
@@ -250,15 +256,34 @@ __device__ __forceinline__ unsigned long long block_sum(unsigned long long v, un
 // dispatched round-robin over the 8 XCDs, so consecutive slices are given to workgroups on the
 // same XCD: neighbouring frontier cells (and their neighbour reads and claims) then share one L2.
 __device__ __forceinline__ void frontier_slice(uint32_t count, uint32_t cap, uint32_t b, uint32_t nb,
-                                               uint32_t &begin, uint32_t &end, uint32_t *slice_out = nullptr) {
+                                               uint32_t &begin, uint32_t &end) {
     const uint32_t n = count < cap ? count : cap;  // an overflowed list is flagged, never over-read
     const uint32_t logical = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
-    if (slice_out) *slice_out = logical;
     uint32_t chunk = (n + nb - 1) / nb;
     chunk = (chunk + kBlock - 1) / kBlock * kBlock;
     const unsigned long long b0 = static_cast<unsigned long long>(logical) * chunk;
     begin = b0 >= n ? n : static_cast<uint32_t>(b0);
     end = (b0 + chunk >= n) ? n : static_cast<uint32_t>(b0 + chunk);
+}
+
+// Wave-aggregated append: every lane with `want` gets its own slot of the list behind `counter`
+// (global or LDS), at one atomic per wave. Every lane of the wave must reach the call.
+__device__ __forceinline__ uint32_t wave_append(bool want, uint32_t *counter) {
+    const unsigned long long b = __ballot(want);
+    if (b == 0) return 0;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll(static_cast<long long>(b)) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, static_cast<uint32_t>(__popcll(b)));
+    base = __shfl(base, leader, 64);
+    return base + static_cast<uint32_t>(__popcll(b & ((1ull << lane) - 1)));
+}
+
+__device__ __forceinline__ void set_active_bit(const DomainView &d, uint32_t q) {
+    atomicOr(&d.bits[q >> 5], 1u << (q & 31));
+}
+__device__ __forceinline__ void clear_active_bit(const DomainView &d, uint32_t q) {
+    atomicAnd(&d.bits[q >> 5], ~(1u << (q & 31)));
 }
 
 // Every body below works on one domain with workgroups b = 0 .. nb-1; the plain launches use the
@@ -270,21 +295,31 @@ __device__ __forceinline__ void nucleate_body(const DomainView &d, int p, int la
     const int cycle = ctl->cycle[p];
     if (b == 0 && threadIdx.x == 0) {
         ctl->cycle[1 - p] = cycle + 1;
-        ctl->serial[1 - p] = ctl->serial[p] + 1;
         ctl->count[1 - p] = 0;
+        ctl->ncand = 0;
     }
     unsigned long long nn = 0, act = 0;
     for (int i = b * kBlock + threadIdx.x; i < d.nnuclei; i += nb * kBlock) {
         const uint32_t q = static_cast<uint32_t>(d.nloc[i]);
         if (!(cycle >= d.ntime[i] && d.ct[q] == kLiquid && d.lid[q] <= layer)) continue;
-        const Coord c = decode(q, d.X, d.Y);
+        const Grid gr = grid_of(d);
+        const Coord c = decode(q, gr);
         if (c.x > 0 && c.x < d.X - 1 && c.y > 0 && c.y < d.Y - 1) nn++;
         d.dl[q] = 0.01f;
+        d.thr[q] = 0.0f;
         d.cx[q] = (c.x + d.xoff) + 0.5;
         d.cy[q] = (c.y + d.yoff) + 0.5;
         d.cz[q] = c.z + 0.5;
+        d.lm[q] = kLmUnknown;
         d.ct[q] = kActive | kSeedBit;
-        if (d.bits) atomicOr(&d.bits[q >> 5], 1u << (q & 31));
+        set_active_bit(d, q);
+        // q is no longer liquid for any neighbour
+        const uint32_t valid = valid_neighbours(c, gr.X, gr.Y, gr.nzA);
+#pragma unroll
+        for (int l = 0; l < 26; l++)
+            if ((valid >> l) & 1u) atomicAnd(&d.lm[q + gr.off[l]], ~(1ull << kOpposite.v[l]));
+        atomicMin(&ctl->rlo[p], q);
+        atomicMax(&ctl->rhi[p], q);
         act++;
         const uint32_t slot = atomicAdd(&ctl->count[p], 1u);
         if (slot < d.list_cap)
@@ -296,228 +331,269 @@ __device__ __forceinline__ void nucleate_body(const DomainView &d, int p, int la
     if (act) atomicAdd(&d.stats->activated, act);
 }
 
-// Pass A. All per-cell loads (own state, octahedron, the 26 neighbour types) are issued before any
-// of them is used, so a frontier cell costs one memory round trip; a neighbour outside the slab
-// loads the cell's own (active, never liquid) type instead and is masked out.
+// Pass A, one frontier cell per lane: the liquidus-time gate, growth, and the claims.
+// The liquid neighbours come from the cell's own mask (lm), kept current by the cells that leave
+// the liquid state; a cell whose mask is not yet known (new this step) reads the 26 neighbour
+// types once. A cell past its liquidus time with no liquid neighbour turns solid here (no capture
+// of this step can give it one).
+// thr[j] is a lower bound on the critical length of every neighbour that is still liquid: the
+// smallest critical length (or a proven lower bound of it) over the liquid neighbours the cell did
+// not claim when it last evaluated them. Liquid sets only shrink and a cell's octahedron centre is
+// fixed, so while the half-diagonal stays below thr no neighbour can be captured and the step
+// needs neither the octahedron nor any critical length. A new active cell starts at thr = 0.
+// Cells with claims are appended to the claim lists inside the workgroup's own frontier slice (an
+// LDS counter, no global atomics), and the slice's count goes to slice_claims[b] for pass B.
 __device__ __forceinline__ void claim_body(const DomainView &d, int p, int layer, uint32_t b, uint32_t nb) {
     __shared__ unsigned long long sh[kBlock / 64];
-    const StepCtl *ctl = d.ctl;
+    __shared__ uint32_t nclaim;
+    StepCtl *ctl = d.ctl;
     if (ctl->done) return;
+    if (threadIdx.x == 0) nclaim = 0;
+    __syncthreads();
     const int cycle = ctl->cycle[p];
-    const unsigned long long stamp = static_cast<unsigned long long>(0xFFFFFFFFu - ctl->serial[p]) << 32;
     const int njumps = ctl->njumps;
     uint32_t begin, end;
     frontier_slice(ctl->count[p], d.list_cap, b, nb, begin, end);
     const uint32_t *list = d.list[p];
     const Grid gr = grid_of(d);
-    unsigned long long front = 0;
-    for (uint32_t i = begin + threadIdx.x; i < end; i += kBlock) {
-        const uint32_t j = list[i];
-        const int t = d.cts[j];
-        const int8_t lay = d.lid[j];
-        const float dl0 = d.dl[j];
-        const uint8_t own = d.ct[j];
-        const int h = d.gid[j];
-        const double cx = d.cx[j], cy = d.cy[j], cz = d.cz[j];
-        const Coord c = decode(j, gr);
-        const uint32_t valid = valid_neighbours(c, gr.X, gr.Y, gr.nzA);
-        uint32_t liquid = 0;
+    unsigned long long front = 0, nwait = 0, nsol = 0;
+    for (uint32_t i0 = begin; i0 < end; i0 += kBlock) {
+        const uint32_t i = i0 + threadIdx.x;
+        uint32_t j = 0, claimed = 0;
+        if (i < end) {
+            j = list[i];
+            const int t = d.cts[j];
+            const int8_t lay = d.lid[j];
+            const float dl0 = d.dl[j];
+            const float thr = d.thr[j];
+            unsigned long long w = d.lm[j];
+            if (w & kLmUnknown) {
+                const Coord c = decode(j, gr);
+                const uint32_t valid = valid_neighbours(c, gr.X, gr.Y, gr.nzA);
+                unsigned long long liq = 0;
 #pragma unroll
-        for (int l = 0; l < 26; l++) {
-            const uint32_t q = ((valid >> l) & 1u) ? j + gr.off[l] : j;
-            liquid |= static_cast<uint32_t>(d.ct[q] == kLiquid) << l;
-        }
-        if (!(cycle > t && lay <= layer)) {
-            d.mask[i] = kWaitMask;
-            continue;
-        }
-        front++;
-        int k = executed_since(t, cycle, ctl, njumps);
-        k = k < 0 ? 0 : (k < d.ngrow - 1 ? k : d.ngrow - 1);
-        const float dlj = static_cast<float>(static_cast<double>(dl0) + d.grow[k]);
-        d.dl[j] = dlj;
-        d.mask[i] = liquid;
-        if (!liquid) continue;
-        int o = orientation_of(h);
-        if (o < 0) {
-            atomicOr(&d.ctl->error, 2);
-            o = 0;
-        }
-        const unsigned long long key = stamp | j;
-        if (own & kSeedBit) {
-            const float *tab = d.seed_cdl + 26 * o;
-            for (uint32_t m = liquid; m; m &= m - 1) {
-                const int l = __builtin_ctz(m);
-                if (dlj >= tab[l]) atomicMin(&d.win[j + gr.off[l]], key);
+                for (int l = 0; l < 26; l++) {
+                    const uint32_t q = ((valid >> l) & 1u) ? j + gr.off[l] : j;
+                    liq |= static_cast<unsigned long long>(d.ct[q] == kLiquid) << l;
+                }
+                w = liq | (liq << 32);
+                d.lm[j] = w;
+            }
+            else if ((w & kLmNow) != (w >> 32)) {
+                w = (w & kLmNow) | ((w & kLmNow) << 32);
+                d.lm[j] = w;
+            }
+            const uint32_t liquid = static_cast<uint32_t>(w & kLmNow);
+            if (!(cycle > t && lay <= layer)) {
+                nwait++;
+            }
+            else {
+                front++;
+                int k = executed_since(t, cycle, ctl, njumps);
+                k = k < 0 ? 0 : (k < d.ngrow - 1 ? k : d.ngrow - 1);
+                const float dlj = static_cast<float>(static_cast<double>(dl0) + d.grow[k]);
+                d.dl[j] = dlj;
+                if (!liquid) {
+                    d.ct[j] = kSolid;
+                    clear_active_bit(d, j);
+                    nsol++;
+                }
+                else if (dlj >= thr) {
+                    int o = orientation_of(d.gid[j]);
+                    if (o < 0) {
+                        atomicOr(&ctl->error, 2);
+                        o = 0;
+                    }
+                    float next = __builtin_inff();
+                    if (d.ct[j] & kSeedBit) {
+                        const float *tab = d.seed_cdl + 26 * o;
+                        for (uint32_t m = liquid; m; m &= m - 1) {
+                            const int l = __builtin_ctz(m);
+                            const float cdl = tab[l];
+                            if (dlj >= cdl) {
+                                atomicMin(&d.win[j + gr.offset(l)], j);
+                                claimed |= 1u << l;
+                            }
+                            else {
+                                next = fminf(next, cdl);
+                            }
+                        }
+                    }
+                    else {
+                        const Coord c = decode(j, gr);
+                        const double cx = d.cx[j], cy = d.cy[j], cz = d.cz[j];
+                        const Faces f = faces_of(d.guv + 9 * o);
+                        const double xp = (c.x + d.xoff) + 0.5, yp = (c.y + d.yoff) + 0.5, zp = c.z + 0.5;
+                        // The critical length of a neighbour at distance |e| from the centre is at
+                        // least sqrt(reach)|e| (reach from the orientation's deviation from
+                        // orthonormality, with a margin far above rounding): below that the exact
+                        // length is not needed, and the bound, rounded down to float, stands in
+                        // for it in thr.
+                        const double reach = d.reach[o];
+                        const double dl2 = static_cast<double>(dlj) * static_cast<double>(dlj);
+                        for (uint32_t m = liquid; m; m &= m - 1) {
+                            const int l = __builtin_ctz(m);
+                            const double ex = xp + nb_dx(l) - cx, ey = yp + nb_dy(l) - cy, ez = zp + nb_dz(l) - cz;
+                            const double r2 = reach * (ex * ex + ey * ey + ez * ez);
+                            if (dl2 < r2) {
+                                next = fminf(next, __double2float_rd(sqrt(r2)));
+                                continue;
+                            }
+                            const float cdl = critical_length(f, xp, yp, zp, cx, cy, cz, l);
+                            if (dlj >= cdl) {
+                                atomicMin(&d.win[j + gr.offset(l)], j);
+                                claimed |= 1u << l;
+                            }
+                            else {
+                                next = fminf(next, cdl);
+                            }
+                        }
+                    }
+                    d.thr[j] = next;
+                }
             }
         }
-        else {
-            const Faces f = faces_of(d.guv + 9 * o);
-            const double xp = (c.x + d.xoff) + 0.5, yp = (c.y + d.yoff) + 0.5, zp = c.z + 0.5;
-            // A neighbour whose distance from the centre bounds its critical length above the
-            // half-diagonal cannot be reached this step; only the others need the exact length.
-            const double reach = d.reach[o];
-            const double dl2 = static_cast<double>(dlj) * static_cast<double>(dlj);
-            for (uint32_t m = liquid; m; m &= m - 1) {
-                const int l = __builtin_ctz(m);
-                const double ex = xp + kNbX[l] - cx, ey = yp + kNbY[l] - cy, ez = zp + kNbZ[l] - cz;
-                if (dl2 < reach * (ex * ex + ey * ey + ez * ez)) continue;
-                if (dlj >= critical_length(f, xp, yp, zp, cx, cy, cz, l)) atomicMin(&d.win[j + gr.off[l]], key);
-            }
+        const uint32_t slot = begin + wave_append(claimed != 0, &nclaim);
+        if (claimed) {
+            d.claim_cell[slot] = j;
+            d.claim_bits[slot] = claimed;
         }
     }
-    const unsigned long long tot = block_sum(front, sh);
-    if (threadIdx.x == 0 && tot) atomicAdd(&d.stats->frontier, tot);
+    __syncthreads();
+    if (threadIdx.x == 0) d.slice_claims[b] = nclaim;
+    unsigned long long t = block_sum(front, sh);
+    if (threadIdx.x == 0 && t) atomicAdd(&d.stats->frontier, t);
+    t = block_sum(nwait, sh);
+    if (threadIdx.x == 0 && t) atomicAdd(&d.stats->waiting, t);
+    t = block_sum(nsol, sh);
+    if (threadIdx.x == 0 && t) atomicAdd(&d.stats->solidified, t);
 }
 
-__device__ __forceinline__ void maybe_stop(const DomainView &d, StepCtl *ctl, int p);
-
-// Pass B. Per frontier entry the outcome is written back over its mask as (keep << 30 | captured
-// neighbours); every workgroup stores the number of next-frontier entries its slice produces so
-// that k_emit can lay the next frontier out in the same order as the current one.
+// Pass B over the cells with claims, workgroup b over the claims of slice b of pass A (the same
+// grid): a claim whose word holds the claimant's own index is won
+// (the lowest index that reached the cell), and the winner builds the captured cell's octahedron
+// (src/CAupdate.cpp:353-563, 717). The captured cell then leaves the liquid masks of its
+// neighbours; a neighbour whose mask this empties is a solidification candidate of this step,
+// appended (per wave) to the candidate list held in the other frontier buffer.
 __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t b, uint32_t nb) {
     __shared__ unsigned long long sh[kBlock / 64];
     StepCtl *ctl = d.ctl;
     if (ctl->done) return;
-    const unsigned long long stamp = static_cast<unsigned long long>(0xFFFFFFFFu - ctl->serial[p]) << 32;
-    uint32_t begin, end, slice;
-    frontier_slice(ctl->count[p], d.list_cap, b, nb, begin, end, &slice);
-    const uint32_t *list = d.list[p];
+    uint32_t begin, end;
+    frontier_slice(ctl->count[p], d.list_cap, b, nb, begin, end);
+    const uint32_t n = begin + d.slice_claims[b];
     const Grid gr = grid_of(d);
-    unsigned long long ncap = 0, nsol = 0, nwait = 0, produced = 0;
-    for (uint32_t i = begin + threadIdx.x; i < end; i += kBlock) {
-        const uint32_t j = list[i];
-        const uint32_t m = d.mask[i];
-        uint32_t captured = 0, keep = 0;
-        if (m == kWaitMask) {
-            keep = 1;
-            nwait++;
-        }
-        else {
-            // Claims on every liquid neighbour, loaded together (masked lanes reload win[j]).
-#pragma unroll
-            for (int l = 0; l < 26; l++) {
-                const uint32_t on = (m >> l) & 1u;
-                const unsigned long long w = d.win[on ? j + gr.off[l] : j];
-                const uint32_t now = (w & 0xFFFFFFFF00000000ull) == stamp;
-                const uint32_t wj = static_cast<uint32_t>(w);
-                // claimed this step: liquid for j only if the winner comes later in index order;
-                // unclaimed: still liquid
-                keep |= on & (now ? static_cast<uint32_t>(wj >= j) : 1u);
-                captured |= (on & now & static_cast<uint32_t>(wj == j)) << l;
-            }
-            if (!keep) {
-                d.ct[j] = kSolid;
-                if (d.bits) atomicAnd(&d.bits[j >> 5], ~(1u << (j & 31)));
-                nsol++;
+    uint32_t *cand = d.list[1 - p];
+    const int lane = threadIdx.x & 63;
+    unsigned long long ncap = 0;
+    for (uint32_t i0 = begin; i0 < n; i0 += kBlock) {
+        const uint32_t i = i0 + threadIdx.x;
+        uint32_t j = 0, todo = 0;
+        if (i < n) {
+            j = d.claim_cell[i];
+            for (uint32_t m = d.claim_bits[i]; m; m &= m - 1) {
+                const int l = __builtin_ctz(m);
+                todo |= static_cast<uint32_t>(d.win[j + gr.offset(l)] == j) << l;
             }
         }
-        if (captured) {
-            const int h = d.gid[j];
+        Coord c{0, 0, 0};
+        int h = 0;
+        const float *g = d.guv;
+        double cxold = 0, cyold = 0, czold = 0;
+        if (todo) {
+            c = decode(j, gr);
+            h = d.gid[j];
             int o = orientation_of(h);
             if (o < 0) {
                 atomicOr(&ctl->error, 2);
                 o = 0;
             }
-            const float *g = d.guv + 9 * o;
-            const Coord c = decode(j, gr);
-            const double cxold = static_cast<float>(d.cx[j]);
-            const double cyold = static_cast<float>(d.cy[j]);
-            const double czold = static_cast<float>(d.cz[j]);
-            for (uint32_t r = captured; r; r &= r - 1) {
-                const int l = __builtin_ctz(r);
-                const uint32_t q = j + gr.off[l];
-                const double xp = (c.x + d.xoff + kNbX[l]) + 0.5;
-                const double yp = (c.y + d.yoff + kNbY[l]) + 0.5;
-                const double zp = (c.z + kNbZ[l]) + 0.5;
+            g = d.guv + 9 * o;
+            cxold = static_cast<float>(d.cx[j]);
+            cyold = static_cast<float>(d.cy[j]);
+            czold = static_cast<float>(d.cz[j]);
+        }
+        // One capture per lane per round; the rounds are wave-uniform so that the candidate
+        // pushes of a round share one atomic.
+        while (__any(todo != 0)) {
+            uint32_t q = 0, emptied = 0;
+            if (todo) {
+                const int l = __builtin_ctz(todo);
+                todo &= todo - 1;
+                q = j + gr.offset(l);
+                const double xp = (c.x + d.xoff + nb_dx(l)) + 0.5;
+                const double yp = (c.y + d.yoff + nb_dy(l)) + 0.5;
+                const double zp = (c.z + nb_dz(l)) + 0.5;
                 const Octahedron oc = new_octahedron(g, cxold, cyold, czold, xp, yp, zp);
                 d.gid[q] = h;
                 d.dl[q] = static_cast<float>(oc.dl);
+                d.thr[q] = 0.0f;
                 d.cx[q] = oc.cx;
                 d.cy[q] = oc.cy;
                 d.cz[q] = oc.cz;
+                d.lm[q] = kLmUnknown;
                 d.ct[q] = kActive;
-                if (d.bits) atomicOr(&d.bits[q >> 5], 1u << (q & 31));
+                set_active_bit(d, q);
                 ncap++;
+                const Coord cq{c.x + nb_dx(l), c.y + nb_dy(l), c.z + nb_dz(l)};
+                const uint32_t valid = valid_neighbours(cq, gr.X, gr.Y, gr.nzA);
+#pragma unroll
+                for (int k = 0; k < 26; k++) {
+                    if ((valid >> k) & 1u) {
+                        const unsigned long long bit = 1ull << kOpposite.v[k];
+                        const unsigned long long old = atomicAnd(&d.lm[q + gr.off[k]], ~bit);
+                        emptied |= static_cast<uint32_t>((old & (kLmUnknown | kLmNow)) == bit) << k;
+                    }
+                }
+            }
+            const uint32_t cnt = __builtin_popcount(emptied);
+            uint32_t tot;
+            const uint32_t incl = wave_inclusive_scan(cnt, tot);
+            if (tot) {
+                uint32_t base = 0;
+                if (lane == 63) base = atomicAdd(&ctl->ncand, tot);
+                base = __shfl(base, 63, 64);
+                uint32_t pos = base + incl - cnt;
+                for (uint32_t r = emptied; r; r &= r - 1, pos++)
+                    if (pos < d.list_cap) cand[pos] = q + gr.offset(__builtin_ctz(r));
             }
         }
-        d.mask[i] = (keep << 30) | captured;
-        produced += keep + __builtin_popcount(captured);
     }
-    unsigned long long t = block_sum(produced, sh);
-    if (threadIdx.x == 0) d.slice_out[slice] = static_cast<uint32_t>(t);
-    t = block_sum(ncap, sh);
+    const unsigned long long t = block_sum(ncap, sh);
     if (threadIdx.x == 0 && t) atomicAdd(&d.stats->captures, t);
-    t = block_sum(nsol, sh);
-    if (threadIdx.x == 0 && t) atomicAdd(&d.stats->solidified, t);
-    t = block_sum(nwait, sh);
-    if (threadIdx.x == 0 && t) atomicAdd(&d.stats->waiting, t);
 }
 
-// Writes the next frontier: slice s's entries start after those of slices 0 .. s-1, and within a
-// slice each entry's survivor comes first, then its captured neighbours in neighbour order. The
-// front therefore keeps the spatial order of the current list (a column-ordered front stays
-// column-ordered), which keeps the per-cell reads of the next step coalesced.
-// With autostop set, the workgroup of slice 0 also ends the domain at a check step (cycle a
-// multiple of 1000) that finds no liquid left: liquid cells only ever leave the liquid state by
-// capture or nucleation, so the reference's superheated + undercooled count
-// (src/CAupdate.cpp:738-783) is the initial liquid count minus both.
-__device__ __forceinline__ void emit_body(const DomainView &d, int p, uint32_t b, uint32_t nb) {
+// Pass C over the candidates: a cell whose every claim-time liquid neighbour was captured in this
+// step by a lower-indexed cell has no liquid neighbour left when the reference visits it in index
+// order, so it turns solid now (src/CAupdate.cpp:333-334, 722-725); a capture by itself or by a
+// later cell leaves it active. A cell not past its liquidus time is not visited this step.
+__device__ __forceinline__ void settle_body(const DomainView &d, int p, int layer, uint32_t b, uint32_t nb) {
     __shared__ unsigned long long sh[kBlock / 64];
-    __shared__ uint32_t wave_tot[kBlock / 64];
     StepCtl *ctl = d.ctl;
     if (ctl->done) return;
-    uint32_t begin, end, slice;
-    frontier_slice(ctl->count[p], d.list_cap, b, nb, begin, end, &slice);
-    unsigned long long before = 0, all = 0;
-    for (uint32_t s = threadIdx.x; s < nb; s += kBlock) {
-        const uint32_t v = d.slice_out[s];
-        all += v;
-        if (s < slice) before += v;
-    }
-    before = block_sum(before, sh);
-    all = block_sum(all, sh);
-    __shared__ unsigned long long base_sh;
-    if (threadIdx.x == 0) {
-        base_sh = before;
-        if (slice == 0) {
-            ctl->count[1 - p] = static_cast<uint32_t>(all < d.list_cap ? all : d.list_cap);
-            if (all > d.list_cap) atomicOr(&ctl->error, 1);
-            maybe_stop(d, ctl, p);
-        }
-    }
-    __syncthreads();
-    unsigned long long base = base_sh;
-    const uint32_t *list = d.list[p];
-    uint32_t *next = d.list[1 - p];
+    const int cycle = ctl->cycle[p];
+    const uint32_t n = ctl->ncand < d.list_cap ? ctl->ncand : d.list_cap;
+    const uint32_t *cand = d.list[1 - p];
     const Grid gr = grid_of(d);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (uint32_t i0 = begin; i0 < end; i0 += kBlock) {
-        const uint32_t i = i0 + threadIdx.x;
-        uint32_t j = 0, m = 0;
-        if (i < end) {
-            j = list[i];
-            m = d.mask[i];
+    unsigned long long nsol = 0;
+    for (uint32_t i = b * kBlock + threadIdx.x; i < n; i += nb * kBlock) {
+        const uint32_t r = cand[i];
+        if ((d.ct[r] & kTypeMask) != kActive) continue;
+        if (!(cycle > d.cts[r] && d.lid[r] <= layer)) continue;
+        bool solid = true;
+        for (uint32_t m = static_cast<uint32_t>(d.lm[r] >> 32); m; m &= m - 1)
+            if (d.win[r + gr.offset(__builtin_ctz(m))] >= r) {
+                solid = false;
+                break;
+            }
+        if (solid) {
+            d.ct[r] = kSolid;
+            clear_active_bit(d, r);
+            nsol++;
         }
-        const uint32_t keep = (m >> 30) & 1u, captured = m & ((1u << 26) - 1);
-        const uint32_t mine = keep + __builtin_popcount(captured);
-        uint32_t wtot;
-        const uint32_t incl = wave_inclusive_scan(mine, wtot);
-        if (lane == 63) wave_tot[wave] = wtot;
-        __syncthreads();
-        uint32_t wbase = 0, round = 0;
-        for (int w = 0; w < kBlock / 64; w++) {
-            if (w < wave) wbase += wave_tot[w];
-            round += wave_tot[w];
-        }
-        unsigned long long pos = base + wbase + incl - mine;
-        if (pos + mine <= d.list_cap) {
-            if (keep) next[pos++] = j;
-            for (uint32_t r = captured; r; r &= r - 1) next[pos++] = j + gr.off[__builtin_ctz(r)];
-        }
-        base += round;
-        __syncthreads();
     }
+    const unsigned long long t = block_sum(nsol, sh);
+    if (threadIdx.x == 0 && t) atomicAdd(&d.stats->solidified, t);
 }
 
 // Device-side end of a domain with autostop: at a check step (cycle a multiple of 1000) with no
@@ -534,21 +610,38 @@ __device__ __forceinline__ void maybe_stop(const DomainView &d, StepCtl *ctl, in
     }
 }
 
-// Index-ordered frontier: the next list is every set bit of `bits` in ascending cell index, so a
-// wave's cells are neighbours in the reference's linear order. Three passes over the bitmap:
-// per-chunk counts, one exclusive scan of the chunk counts, and the write.
+// The next frontier is every set bit of `bits` in ascending cell index, so a wave's cells are
+// neighbours in the reference's linear order. Three passes over the bitmap chunks that can hold an
+// active cell: per-chunk counts, one exclusive scan of the chunk counts, and the write.
+// Every active cell of a step lies in [rlo, rhi]: the write sets it to the first and last cell of
+// the new frontier widened by the largest neighbour offset (captures stay within it), and
+// nucleation widens it to each new nucleus.
 constexpr uint32_t kChunkWords = 8 * kBlock;
+constexpr uint32_t kChunkCells = 32 * kChunkWords;
 
-__device__ __forceinline__ void bits_count_body(const DomainView &d, uint32_t b, uint32_t nb) {
+__device__ __forceinline__ bool chunk_range(const DomainView &d, int p, uint32_t &c0, uint32_t &c1) {
+    const uint32_t lo = d.ctl->rlo[p];
+    uint32_t hi = d.ctl->rhi[p];
+    if (lo > hi) return false;
+    if (hi >= static_cast<uint64_t>(d.ncell)) hi = static_cast<uint32_t>(d.ncell - 1);
+    c0 = lo / kChunkCells;
+    c1 = hi / kChunkCells;
+    return true;
+}
+
+__device__ __forceinline__ void bits_count_body(const DomainView &d, int p, uint32_t b, uint32_t nb) {
     __shared__ unsigned long long sh[kBlock / 64];
     if (d.ctl->done) return;
+    uint32_t c0, c1;
+    if (!chunk_range(d, p, c0, c1)) return;
     const uint32_t nwords = static_cast<uint32_t>((d.ncell + 31) / 32);
-    for (uint32_t c = b; c < d.nchunks; c += nb) {
-        const uint32_t w0 = c * kChunkWords + threadIdx.x * 8;
+    for (uint32_t c = c0 + b; c <= c1; c += nb) {
         uint32_t cnt = 0;
 #pragma unroll
-        for (int k = 0; k < 8; k++)
-            if (w0 + k < nwords) cnt += __builtin_popcount(d.bits[w0 + k]);
+        for (uint32_t k = 0; k < kChunkWords / kBlock; k++) {
+            const uint32_t wi = c * kChunkWords + k * kBlock + threadIdx.x;
+            if (wi < nwords) cnt += __builtin_popcount(d.bits[wi]);
+        }
         const unsigned long long t = block_sum(cnt, sh);
         if (threadIdx.x == 0) d.chunk_sum[c] = static_cast<uint32_t>(t);
     }
@@ -558,10 +651,14 @@ __device__ __forceinline__ void bits_scan_body(const DomainView &d, int p) {
     __shared__ uint32_t part[kBlock];
     StepCtl *ctl = d.ctl;
     if (ctl->done) return;
-    const uint32_t per = (d.nchunks + kBlock - 1) / kBlock;
-    const uint32_t c0 = threadIdx.x * per;
+    uint32_t c0 = 1, c1 = 0;
+    const bool any = chunk_range(d, p, c0, c1);
+    const uint32_t nc = any ? c1 - c0 + 1 : 0;
+    const uint32_t per = (nc + kBlock - 1) / kBlock;
+    const uint32_t s0 = c0 + threadIdx.x * per;
+    const uint32_t s1 = any ? (s0 + per <= c1 + 1 ? s0 + per : c1 + 1) : 0;
     uint32_t mine = 0;
-    for (uint32_t c = c0; c < c0 + per && c < d.nchunks; c++) mine += d.chunk_sum[c];
+    for (uint32_t c = s0; c < s1; c++) mine += d.chunk_sum[c];
     part[threadIdx.x] = mine;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -573,44 +670,65 @@ __device__ __forceinline__ void bits_scan_body(const DomainView &d, int p) {
         }
         ctl->count[1 - p] = static_cast<uint32_t>(run < d.list_cap ? run : d.list_cap);
         if (run > d.list_cap) atomicOr(&ctl->error, 1);
+        if (run == 0) {
+            ctl->rlo[1 - p] = 0xFFFFFFFFu;
+            ctl->rhi[1 - p] = 0;
+        }
         maybe_stop(d, ctl, p);
     }
     __syncthreads();
     uint32_t run = part[threadIdx.x];
-    for (uint32_t c = c0; c < c0 + per && c < d.nchunks; c++) {
+    for (uint32_t c = s0; c < s1; c++) {
         const uint32_t v = d.chunk_sum[c];
         d.chunk_sum[c] = run;
         run += v;
     }
 }
 
+// Chunk c is written in kChunkWords / kBlock rounds of one bitmap word per thread (consecutive
+// threads, consecutive words). A round's cells are staged in LDS at their offsets within the round
+// and then stored to the list with consecutive lanes on consecutive entries.
 __device__ __forceinline__ void bits_write_body(const DomainView &d, int p, uint32_t b, uint32_t nb) {
     __shared__ uint32_t wave_tot[kBlock / 64];
-    if (d.ctl->done) return;
+    __shared__ uint32_t stage[32 * kBlock];
+    StepCtl *ctl = d.ctl;
+    if (ctl->done) return;
+    uint32_t c0, c1;
+    if (!chunk_range(d, p, c0, c1)) return;
     const uint32_t nwords = static_cast<uint32_t>((d.ncell + 31) / 32);
+    const uint32_t total = ctl->count[1 - p];
+    const uint32_t reach = static_cast<uint32_t>(d.X) * static_cast<uint32_t>(d.Y) + static_cast<uint32_t>(d.Y) + 1;
+    const uint32_t last = static_cast<uint32_t>(d.ncell - 1);
     uint32_t *next = d.list[1 - p];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (uint32_t c = b; c < d.nchunks; c += nb) {
-        const uint32_t w0 = c * kChunkWords + threadIdx.x * 8;
-        uint32_t w[8], cnt = 0;
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-            w[k] = (w0 + k < nwords) ? d.bits[w0 + k] : 0u;
-            cnt += __builtin_popcount(w[k]);
+    for (uint32_t c = c0 + b; c <= c1; c += nb) {
+        uint32_t base = d.chunk_sum[c];
+        for (uint32_t r = 0; r < kChunkWords / kBlock; r++) {
+            const uint32_t wi = c * kChunkWords + r * kBlock + threadIdx.x;
+            const uint32_t w = wi < nwords ? d.bits[wi] : 0u;
+            const uint32_t cnt = __builtin_popcount(w);
+            uint32_t wtot;
+            const uint32_t incl = wave_inclusive_scan(cnt, wtot);
+            if (lane == 63) wave_tot[wave] = wtot;
+            __syncthreads();
+            uint32_t o = 0, rtot = 0;
+            for (int q = 0; q < kBlock / 64; q++) {
+                if (q < wave) o += wave_tot[q];
+                rtot += wave_tot[q];
+            }
+            o += incl - cnt;
+            for (uint32_t m = w; m; m &= m - 1) stage[o++] = wi * 32u + __builtin_ctz(m);
+            __syncthreads();
+            for (uint32_t k = threadIdx.x; k < rtot; k += kBlock) {
+                const uint32_t pos = base + k;
+                const uint32_t q = stage[k];
+                if (pos < d.list_cap) next[pos] = q;
+                if (pos == 0) ctl->rlo[1 - p] = q > reach ? q - reach : 0;
+                if (pos + 1 == total) ctl->rhi[1 - p] = (last - q > reach) ? q + reach : last;
+            }
+            base += rtot;
+            __syncthreads();
         }
-        uint32_t wtot;
-        const uint32_t incl = wave_inclusive_scan(cnt, wtot);
-        if (lane == 63) wave_tot[wave] = wtot;
-        __syncthreads();
-        uint32_t wbase = 0;
-        for (int q = 0; q < wave; q++) wbase += wave_tot[q];
-        unsigned long long pos = static_cast<unsigned long long>(d.chunk_sum[c]) + wbase + incl - cnt;
-        if (pos + cnt <= d.list_cap) {
-#pragma unroll
-            for (int k = 0; k < 8; k++)
-                for (uint32_t r = w[k]; r; r &= r - 1) next[pos++] = (w0 + k) * 32u + __builtin_ctz(r);
-        }
-        __syncthreads();
     }
 }
 
@@ -622,8 +740,12 @@ __global__ __launch_bounds__(kBlock) void k_claim(DomainView d, int p, int layer
     claim_body(d, p, layer, blockIdx.x, gridDim.x);
 }
 __global__ __launch_bounds__(kBlock) void k_apply(DomainView d, int p) { apply_body(d, p, blockIdx.x, gridDim.x); }
-__global__ __launch_bounds__(kBlock) void k_emit(DomainView d, int p) { emit_body(d, p, blockIdx.x, gridDim.x); }
-__global__ __launch_bounds__(kBlock) void k_bits_count(DomainView d) { bits_count_body(d, blockIdx.x, gridDim.x); }
+__global__ __launch_bounds__(kBlock) void k_settle(DomainView d, int p, int layer) {
+    settle_body(d, p, layer, blockIdx.x, gridDim.x);
+}
+__global__ __launch_bounds__(kBlock) void k_bits_count(DomainView d, int p) {
+    bits_count_body(d, p, blockIdx.x, gridDim.x);
+}
 __global__ __launch_bounds__(kBlock) void k_bits_scan(DomainView d, int p) { bits_scan_body(d, p); }
 __global__ __launch_bounds__(kBlock) void k_bits_write(DomainView d, int p) {
     bits_write_body(d, p, blockIdx.x, gridDim.x);
@@ -643,21 +765,21 @@ __global__ __launch_bounds__(kBlock) void k_apply_n(const DomainView *v, int p) 
     const DomainView &d = v[blockIdx.y];
     if (d.ctl) apply_body(d, p, blockIdx.x, gridDim.x);
 }
-__global__ __launch_bounds__(kBlock) void k_emit_n(const DomainView *v, int p) {
+__global__ __launch_bounds__(kBlock) void k_settle_n(const DomainView *v, int p, int layer) {
     const DomainView &d = v[blockIdx.y];
-    if (d.ctl && !d.index_order) emit_body(d, p, blockIdx.x, gridDim.x);
+    if (d.ctl) settle_body(d, p, layer, blockIdx.x, gridDim.x);
 }
-__global__ __launch_bounds__(kBlock) void k_bits_count_n(const DomainView *v) {
+__global__ __launch_bounds__(kBlock) void k_bits_count_n(const DomainView *v, int p) {
     const DomainView &d = v[blockIdx.y];
-    if (d.ctl && d.index_order) bits_count_body(d, blockIdx.x, gridDim.x);
+    if (d.ctl) bits_count_body(d, p, blockIdx.x, gridDim.x);
 }
 __global__ __launch_bounds__(kBlock) void k_bits_scan_n(const DomainView *v, int p) {
     const DomainView &d = v[blockIdx.y];
-    if (d.ctl && d.index_order) bits_scan_body(d, p);
+    if (d.ctl) bits_scan_body(d, p);
 }
 __global__ __launch_bounds__(kBlock) void k_bits_write_n(const DomainView *v, int p) {
     const DomainView &d = v[blockIdx.y];
-    if (d.ctl && d.index_order) bits_write_body(d, p, blockIdx.x, gridDim.x);
+    if (d.ctl) bits_write_body(d, p, blockIdx.x, gridDim.x);
 }
 
 __global__ __launch_bounds__(kBlock) void k_check(DomainView d, int cycle, int layer, CheckResult *out) {
@@ -755,50 +877,46 @@ void launch_claim(const DomainView &d, int parity, int layer, int blocks, hipStr
     k_claim<<<blocks, kBlock, 0, s>>>(d, parity, layer);
 }
 void launch_apply(const DomainView &d, int parity, int layer, int blocks, hipStream_t s) {
-    (void)layer;
+    const int cb = static_cast<int>(d.nchunks < static_cast<uint32_t>(blocks) ? d.nchunks : blocks);
     k_apply<<<blocks, kBlock, 0, s>>>(d, parity);
-    if (d.index_order) {
-        const int cb = static_cast<int>(d.nchunks < static_cast<uint32_t>(blocks) ? d.nchunks : blocks);
-        k_bits_count<<<cb, kBlock, 0, s>>>(d);
-        k_bits_scan<<<1, kBlock, 0, s>>>(d, parity);
-        k_bits_write<<<cb, kBlock, 0, s>>>(d, parity);
-    }
-    else {
-        k_emit<<<blocks, kBlock, 0, s>>>(d, parity);
-    }
+    k_settle<<<blocks, kBlock, 0, s>>>(d, parity, layer);
+    k_bits_count<<<cb, kBlock, 0, s>>>(d, parity);
+    k_bits_scan<<<1, kBlock, 0, s>>>(d, parity);
+    k_bits_write<<<cb, kBlock, 0, s>>>(d, parity);
 }
 void launch_step_timed(const DomainView &d, int parity, int layer, int blocks, hipStream_t s, hipEvent_t *ev) {
-    auto rec = [&](int i) { (void)hipEventRecord(ev[i], s); };
-    rec(0);
-    k_nucleate<<<grid_for(d.nnuclei, 1024), kBlock, 0, s>>>(d, parity, layer);
-    rec(1);
-    rec(2);
-    k_claim<<<blocks, kBlock, 0, s>>>(d, parity, layer);
-    rec(3);
-    rec(4);
-    k_apply<<<blocks, kBlock, 0, s>>>(d, parity);
-    rec(5);
     const int cb = static_cast<int>(d.nchunks < static_cast<uint32_t>(blocks) ? d.nchunks : blocks);
-    rec(6);
-    if (d.index_order)
-        k_bits_count<<<cb, kBlock, 0, s>>>(d);
-    else
-        k_emit<<<blocks, kBlock, 0, s>>>(d, parity);
-    rec(7);
-    rec(8);
-    if (d.index_order) k_bits_scan<<<1, kBlock, 0, s>>>(d, parity);
-    rec(9);
-    rec(10);
-    if (d.index_order) k_bits_write<<<cb, kBlock, 0, s>>>(d, parity);
-    rec(11);
+    int e = 0;
+    auto rec = [&]() { (void)hipEventRecord(ev[e++], s); };
+    rec();
+    k_nucleate<<<grid_for(d.nnuclei, 1024), kBlock, 0, s>>>(d, parity, layer);
+    rec();
+    rec();
+    k_claim<<<blocks, kBlock, 0, s>>>(d, parity, layer);
+    rec();
+    rec();
+    k_apply<<<blocks, kBlock, 0, s>>>(d, parity);
+    rec();
+    rec();
+    k_settle<<<blocks, kBlock, 0, s>>>(d, parity, layer);
+    rec();
+    rec();
+    k_bits_count<<<cb, kBlock, 0, s>>>(d, parity);
+    rec();
+    rec();
+    k_bits_scan<<<1, kBlock, 0, s>>>(d, parity);
+    rec();
+    rec();
+    k_bits_write<<<cb, kBlock, 0, s>>>(d, parity);
+    rec();
 }
 void launch_step_n(const DomainView *views, int slots, int parity, int layer, int nuc_blocks, int blocks,
                    hipStream_t s) {
     k_nucleate_n<<<dim3(nuc_blocks, slots), kBlock, 0, s>>>(views, parity, layer);
     k_claim_n<<<dim3(blocks, slots), kBlock, 0, s>>>(views, parity, layer);
     k_apply_n<<<dim3(blocks, slots), kBlock, 0, s>>>(views, parity);
-    k_emit_n<<<dim3(blocks, slots), kBlock, 0, s>>>(views, parity);
-    k_bits_count_n<<<dim3(blocks, slots), kBlock, 0, s>>>(views);
+    k_settle_n<<<dim3(blocks, slots), kBlock, 0, s>>>(views, parity, layer);
+    k_bits_count_n<<<dim3(blocks, slots), kBlock, 0, s>>>(views, parity);
     k_bits_scan_n<<<dim3(1, slots), kBlock, 0, s>>>(views, parity);
     k_bits_write_n<<<dim3(blocks, slots), kBlock, 0, s>>>(views, parity);
 }

@@ -12,8 +12,8 @@ constexpr int kBlock = 256;  // four wave64s per workgroup
 void launch_nucleation(const DomainView &d, int parity, int layer, hipStream_t s);
 // CellCapture pass A: liquidus-time gate, growth, neighbour scan, claims (src/CAupdate.cpp:217-352).
 void launch_claim(const DomainView &d, int parity, int layer, int blocks, hipStream_t s);
-// CellCapture pass B: winners build new octahedra, losers/finished cells solidify, next frontier
-// (src/CAupdate.cpp:353-725).
+// CellCapture pass B: winners build new octahedra, cells left without a liquid neighbour by
+// lower-indexed captures solidify, and the next frontier is written (src/CAupdate.cpp:353-725).
 void launch_apply(const DomainView &d, int parity, int layer, int blocks, hipStream_t s);
 // One step of every domain in a device-resident table of `slots` views (grid row = slot).
 void launch_step_n(const DomainView *views, int slots, int parity, int layer, int nuc_blocks, int blocks,
@@ -22,10 +22,9 @@ void launch_step_n(const DomainView *views, int slots, int parity, int layer, in
 void launch_check(const DomainView &d, int cycle, int layer, CheckResult *out, hipStream_t s);
 // Octahedra of the initially active cells (src/CAinitialize.cpp:1936-2043).
 void launch_seed_init(const DomainView &d, const uint32_t *cells, uint32_t n, hipStream_t s);
-// One step with an event pair around each kernel: nucleation, claim, apply, and the frontier
-// rebuild (count, scan, write for the index order; emit alone for the capture order, the two
-// unused pairs then time nothing). ev holds 2 * kTimedKernels events.
-constexpr int kTimedKernels = 6;
+// One step with an event pair around each kernel: nucleation, claim, apply, settle, and the
+// frontier rebuild (count, scan, write). ev holds 2 * kTimedKernels events.
+constexpr int kTimedKernels = 7;
 void launch_step_timed(const DomainView &d, int parity, int layer, int blocks, hipStream_t s, hipEvent_t *ev);
 // Plain copy of n 16-byte words (HBM calibration).
 void launch_copy(const uint4 *src, uint4 *dst, size_t n, hipStream_t s);

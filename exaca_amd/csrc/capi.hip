@@ -27,12 +27,6 @@ thread_local std::string g_last_error;
 }
 
 namespace exaca {
-// Frontier order of new domains: cell-index order rebuilt from the active bitmap each step
-// (EXACA_FRONTIER_ORDER=index, default) or the capture order kept by k_emit (=capture).
-int frontier_index_order() {
-    const char *v = std::getenv("EXACA_FRONTIER_ORDER");
-    return (v && std::string(v) == "capture") ? 0 : 1;
-}
 std::vector<uint32_t> active_bits(const HostDomain &hd) {
     std::vector<uint32_t> b((static_cast<size_t>(hd.ncell) + 31) / 32, 0u);
     for (uint32_t q : hd.active) b[q >> 5] |= 1u << (q & 31);
@@ -43,7 +37,6 @@ std::vector<uint32_t> active_bits(const HostDomain &hd) {
 namespace {
 
 using exaca::active_bits;
-using exaca::frontier_index_order;
 using exaca::DeviceError;
 using exaca::StateError;
 using exaca::guarded;
@@ -117,20 +110,22 @@ void build_device_state(exaca_state *h) {
     d.cts = h->alloc<int32_t>(n);
     d.lid = h->alloc<int8_t>(n);
     d.dl = h->alloc<float>(n);
+    d.thr = h->alloc<float>(n);
     d.cx = h->alloc<double>(n);
     d.cy = h->alloc<double>(n);
     d.cz = h->alloc<double>(n);
-    d.win = h->alloc<unsigned long long>(n);
+    d.win = h->alloc<uint32_t>(n);
     d.list_cap = static_cast<uint32_t>(n);
     d.list[0] = h->alloc<uint32_t>(n);
     d.list[1] = h->alloc<uint32_t>(n);
-    d.mask = h->alloc<uint32_t>(n);
-    d.slice_out = h->alloc<uint32_t>(h->blocks);
+    d.lm = h->alloc<unsigned long long>(n);
+    d.claim_cell = h->alloc<uint32_t>(n);
+    d.claim_bits = h->alloc<uint32_t>(n);
+    d.slice_claims = h->alloc<uint32_t>(h->blocks);
     const size_t nwords = (n + 31) / 32;
     d.nchunks = static_cast<uint32_t>((nwords + 2047) / 2048);
     d.bits = h->alloc<uint32_t>(nwords);
     d.chunk_sum = h->alloc<uint32_t>(d.nchunks);
-    d.index_order = frontier_index_order();
     d.nnuclei = static_cast<int32_t>(hd.nloc.size());
     int32_t *nloc = h->alloc<int32_t>(hd.nloc.size());
     int32_t *ntime = h->alloc<int32_t>(hd.ntime.size());
@@ -168,10 +163,12 @@ void build_device_state(exaca_state *h) {
     HIP_OK(hipMemcpyAsync(d.cts, hd.cts.data(), n * 4, hipMemcpyHostToDevice, s));
     HIP_OK(hipMemcpyAsync(d.lid, hd.lid.data(), n, hipMemcpyHostToDevice, s));
     HIP_OK(hipMemsetAsync(d.dl, 0, n * 4, s));
+    HIP_OK(hipMemsetAsync(d.thr, 0, n * 4, s));
     HIP_OK(hipMemsetAsync(d.cx, 0, n * 8, s));
     HIP_OK(hipMemsetAsync(d.cy, 0, n * 8, s));
     HIP_OK(hipMemsetAsync(d.cz, 0, n * 8, s));
-    HIP_OK(hipMemsetAsync(d.win, 0xFF, n * 8, s));
+    HIP_OK(hipMemsetAsync(d.win, 0xFF, n * 4, s));
+    HIP_OK(hipMemsetAsync(d.lm, 0xFF, n * 8, s));
     if (!hd.nloc.empty()) {
         HIP_OK(hipMemcpyAsync(nloc, hd.nloc.data(), hd.nloc.size() * 4, hipMemcpyHostToDevice, s));
         HIP_OK(hipMemcpyAsync(ntime, hd.ntime.data(), hd.ntime.size() * 4, hipMemcpyHostToDevice, s));
@@ -190,8 +187,9 @@ void build_device_state(exaca_state *h) {
     StepCtl c;
     std::memset(&c, 0, sizeof c);
     c.cycle[0] = 1;
-    c.serial[0] = 1;
     c.count[0] = static_cast<uint32_t>(hd.active.size());
+    c.rlo[0] = 0;
+    c.rhi[0] = static_cast<uint32_t>(n - 1);
     HIP_OK(hipMemcpyAsync(h->ctl, &c, sizeof c, hipMemcpyHostToDevice, s));
     HIP_OK(hipMemsetAsync(h->stats, 0, sizeof(StepStats), s));
     HIP_OK(hipStreamSynchronize(s));
@@ -340,7 +338,7 @@ exaca_state *create_state(const std::string &input, const std::string &dd, bool 
 }
 
 // Steps one at a time with HIP events around every kernel (no graphs): per-kernel device time of
-// nucleation, claim, apply and the frontier rebuild (emit, or bitmap count / scan / write).
+// nucleation, claim, apply, settle and the frontier rebuild (bitmap count / scan / write).
 void profile(exaca_state *h, int layer, int nsteps, double *ms, int64_t counts[5], int64_t *launches) {
     using exaca::kTimedKernels;
     exaca::StepStats before, after;
@@ -349,7 +347,7 @@ void profile(exaca_state *h, int layer, int nsteps, double *ms, int64_t counts[5
     const int per_batch = 128;
     std::vector<hipEvent_t> ev(2 * kTimedKernels * per_batch);
     for (auto &e : ev) HIP_OK(hipEventCreate(&e));
-    double acc[kTimedKernels] = {0, 0, 0, 0, 0, 0};
+    double acc[kTimedKernels] = {};
     int done = 0, xswitch = 0;
     while (done < nsteps && !xswitch) {
         const int batch = std::min(per_batch, nsteps - done);
@@ -554,7 +552,7 @@ int exaca_profile_steps(exaca_state *h, int layer, int nsteps, double ms[3], int
     });
 }
 
-int exaca_profile_kernels(exaca_state *h, int layer, int nsteps, double ms[6], int64_t counts[5]) {
+int exaca_profile_kernels(exaca_state *h, int layer, int nsteps, double ms[7], int64_t counts[5]) {
     return guarded([&] {
         if (!h || !ms || !counts || nsteps < 0) throw std::invalid_argument("bad argument");
         if (h->pending >= 0) throw StateError("profile_kernels called between nucleation and capture");

@@ -24,7 +24,6 @@
 #include "runtime_util.hpp"
 
 namespace exaca {
-int frontier_index_order();
 std::vector<uint32_t> active_bits(const HostDomain &hd);
 }  // namespace exaca
 
@@ -82,9 +81,10 @@ struct Arena {
 size_t domain_bytes(const exaca::HostDomain &h, size_t ngrow) {
     const size_t n = static_cast<size_t>(h.ncell);
     size_t b = 0;
-    b += up(n) + up(4 * n) + up(4 * n) + up(n) + up(4 * n) + 3 * up(8 * n) + up(8 * n);  // state + win
-    b += 3 * up(4 * n);                                                                  // lists + mask
-    b += up(4 * kMaxSlices) + 2 * up(4 * std::max<size_t>(h.nloc.size(), 1)) + up(8 * ngrow);
+    b += up(n) + up(4 * n) + up(4 * n) + up(n) + 2 * up(4 * n) + 3 * up(8 * n);  // cell state
+    b += up(8 * n) + up(4 * n);                                                  // liquid masks, claims
+    b += 4 * up(4 * n) + up(4 * kMaxSlices);                                     // frontier, claim lists
+    b += 2 * up(4 * std::max<size_t>(h.nloc.size(), 1)) + up(8 * ngrow);
     b += up(sizeof(exaca::StepCtl)) + up(sizeof(exaca::StepStats));
     const size_t nwords = (n + 31) / 32;
     b += up(4 * nwords) + up(4 * ((nwords + 2047) / 2048));
@@ -206,15 +206,18 @@ void admit(exaca_ensemble *e, const exaca_prepared *pr, int *slot_out) {
     d.cts = reinterpret_cast<int32_t *>(carve(4 * n));
     d.lid = reinterpret_cast<int8_t *>(carve(n));
     d.dl = reinterpret_cast<float *>(carve(4 * n));
+    d.thr = reinterpret_cast<float *>(carve(4 * n));
     d.cx = reinterpret_cast<double *>(carve(8 * n));
     d.cy = reinterpret_cast<double *>(carve(8 * n));
     d.cz = reinterpret_cast<double *>(carve(8 * n));
-    d.win = reinterpret_cast<unsigned long long *>(carve(8 * n));
+    d.lm = reinterpret_cast<unsigned long long *>(carve(8 * n));
+    d.win = reinterpret_cast<uint32_t *>(carve(4 * n));
     d.list[0] = reinterpret_cast<uint32_t *>(carve(4 * n));
     d.list[1] = reinterpret_cast<uint32_t *>(carve(4 * n));
-    d.mask = reinterpret_cast<uint32_t *>(carve(4 * n));
+    d.claim_cell = reinterpret_cast<uint32_t *>(carve(4 * n));
+    d.claim_bits = reinterpret_cast<uint32_t *>(carve(4 * n));
+    d.slice_claims = reinterpret_cast<uint32_t *>(carve(4 * kMaxSlices));
     d.list_cap = static_cast<uint32_t>(n);
-    d.slice_out = reinterpret_cast<uint32_t *>(carve(4 * kMaxSlices));
     int32_t *nloc = reinterpret_cast<int32_t *>(carve(4 * std::max<size_t>(hd.nloc.size(), 1)));
     int32_t *ntime = reinterpret_cast<int32_t *>(carve(4 * std::max<size_t>(hd.nloc.size(), 1)));
     double *grow = reinterpret_cast<double *>(carve(8 * pr->grow.size()));
@@ -222,7 +225,6 @@ void admit(exaca_ensemble *e, const exaca_prepared *pr, int *slot_out) {
     d.nchunks = static_cast<uint32_t>((nwords + 2047) / 2048);
     d.bits = reinterpret_cast<uint32_t *>(carve(4 * nwords));
     d.chunk_sum = reinterpret_cast<uint32_t *>(carve(4 * d.nchunks));
-    d.index_order = exaca::frontier_index_order();
     d.ctl = reinterpret_cast<StepCtl *>(carve(sizeof(StepCtl)));
     d.stats = reinterpret_cast<StepStats *>(carve(sizeof(StepStats)));
     d.nnuclei = static_cast<int32_t>(hd.nloc.size());
@@ -245,7 +247,9 @@ void admit(exaca_ensemble *e, const exaca_prepared *pr, int *slot_out) {
     HIP_OK(hipMemcpyAsync(d.cts, hd.cts.data(), 4 * n, hipMemcpyHostToDevice, s));
     HIP_OK(hipMemcpyAsync(d.lid, hd.lid.data(), n, hipMemcpyHostToDevice, s));
     HIP_OK(hipMemsetAsync(d.dl, 0, 4 * n, s));
-    HIP_OK(hipMemsetAsync(d.win, 0xFF, 8 * n, s));
+    HIP_OK(hipMemsetAsync(d.thr, 0, 4 * n, s));
+    HIP_OK(hipMemsetAsync(d.win, 0xFF, 4 * n, s));
+    HIP_OK(hipMemsetAsync(d.lm, 0xFF, 8 * n, s));
     if (!hd.nloc.empty()) {
         HIP_OK(hipMemcpyAsync(nloc, hd.nloc.data(), 4 * hd.nloc.size(), hipMemcpyHostToDevice, s));
         HIP_OK(hipMemcpyAsync(ntime, hd.ntime.data(), 4 * hd.ntime.size(), hipMemcpyHostToDevice, s));
@@ -262,8 +266,9 @@ void admit(exaca_ensemble *e, const exaca_prepared *pr, int *slot_out) {
     StepCtl c;
     std::memset(&c, 0, sizeof c);
     c.cycle[0] = 1;
-    c.serial[0] = 1;
     c.count[0] = static_cast<uint32_t>(hd.active.size());
+    c.rlo[0] = 0;
+    c.rhi[0] = static_cast<uint32_t>(n - 1);
     HIP_OK(hipMemcpyAsync(d.ctl, &c, sizeof c, hipMemcpyHostToDevice, s));
     HIP_OK(hipMemsetAsync(d.stats, 0, sizeof(StepStats), s));
     const int32_t zero = 0;

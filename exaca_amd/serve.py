@@ -26,20 +26,38 @@ def digest(cycle: int, ct: np.ndarray, gid: np.ndarray) -> str:
     return h.hexdigest()
 
 
-def run_sequential(requests, log=None) -> dict:
-    """The plain GPU path, one request after another (host init + run + download each)."""
+def run_sequential(requests, realtime: bool = True, log=None) -> dict:
+    """The plain GPU path, one request after another in arrival order (host init + run + download
+    each). With realtime a request is not started before its arrival time; its latency is its
+    completion time minus its arrival."""
+    order = sorted(requests, key=lambda r: r.arrival)
     t0 = time.perf_counter()
     out = {}
+    finished = {}
     gpu_ms = 0.0
-    for r in requests:
+    for r in order:
+        if realtime:
+            wait = r.arrival - (time.perf_counter() - t0)
+            if wait > 0:
+                time.sleep(wait)
         with Solver(r.deck, seed=r.seed) as sol:
             cycle, ms = sol.run()
             gpu_ms += ms
             out[r.index] = digest(cycle, sol.field(FIELD_CELLTYPE), sol.field(FIELD_GRAINID))
+        finished[r.index] = time.perf_counter() - t0
         if log:
             log(f"sequential {r.index}: {r.nx}x{r.ny}x{r.nz} cycle {cycle} {ms:.1f} ms")
     wall = time.perf_counter() - t0
-    return {"wall_s": wall, "gpu_s": gpu_ms / 1e3, "digests": out, "solves_per_s": len(requests) / wall}
+    lat = np.array([finished[r.index] - (r.arrival if realtime else 0.0) for r in requests])
+    first = min(r.arrival for r in requests) if realtime else 0.0
+    return {
+        "wall_s": wall,
+        "gpu_s": gpu_ms / 1e3,
+        "digests": out,
+        "solves_per_s": len(requests) / (max(finished.values()) - first),
+        "p50_s": float(np.percentile(lat, 50)),
+        "p99_s": float(np.percentile(lat, 99)),
+    }
 
 
 def run_served(requests, *, slots: int = 256, arena_bytes: int = 200 << 30, chunk: int = 50,
