@@ -108,6 +108,7 @@ def main() -> int:
     ap.add_argument("--cpu-cycles", type=int, default=150)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ensemble", type=int, default=64, help="process-map requests (0: skip)")
+    ap.add_argument("--copy-probe-gib", type=int, default=4, help="size of the HBM copy calibration (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs one GPU: the whole CA domain lives on a single MI355X (asked for {args.gpus})",
@@ -122,9 +123,11 @@ def main() -> int:
     init_s = time.perf_counter() - t0
     print(f"[bench] initialised {deck.name}: {sol.X}x{sol.Y}x{sol.nz} cells in {init_s:.1f} s", file=sys.stderr,
           flush=True)
-    copy_bytes = 1 << 32
-    copy_ms = ea.copy_probe(copy_bytes, 10)
-    copy_gbps = 2 * copy_bytes / (copy_ms / 1e3) / 1e9
+    copy_gbps = None
+    if args.copy_probe_gib > 0:
+        copy_bytes = args.copy_probe_gib << 30
+        copy_ms = ea.copy_probe(copy_bytes, 10)
+        copy_gbps = 2 * copy_bytes / (copy_ms / 1e3) / 1e9
 
     if args.warmup > 0:
         sol.run(max_cycles=args.warmup)
@@ -196,6 +199,8 @@ def main() -> int:
             "bytes_per_update": 29,
         },
         "kernel_avg_us": {k: 1e3 * v / nsteps for k, v in kms.items()},
+        "per_step": {k: (q1[k] - q0[k]) / max(steps, 1)
+                     for k in ("frontier", "captures", "solidified", "evaluated", "masks")},
         "copy_GBps": copy_gbps,
         "init_s": init_s,
         "wall_s": wall,

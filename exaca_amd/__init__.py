@@ -290,10 +290,10 @@ class Solver:
         return out
 
     def query(self) -> dict:
-        out = (ctypes.c_int64 * 16)()
+        out = (ctypes.c_int64 * 18)()
         _ok(self._lib.exaca_query(self._h, out))
         keys = ["X", "Y", "nz", "nzA", "nx", "ny", "nuclei", "cycle", "nn", "frontier", "captures",
-                "solidified", "waiting", "list_len", "list_cap", "device_bytes"]
+                "solidified", "waiting", "list_len", "list_cap", "device_bytes", "evaluated", "masks"]
         return dict(zip(keys, list(out)))
 
     def profile_steps(self, nsteps: int, layer: int = 0) -> dict:

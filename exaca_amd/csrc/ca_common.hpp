@@ -1,19 +1,22 @@
 // ca_common.hpp -- types shared by the host runtime and the CDNA4 kernels of the CA time step.
 //
-// Cell-state layout on the GPU (structure of arrays, one entry per cell of the whole local
-// domain, reference linear order p = z*X*Y + x*Y + y with y fastest, src/CAupdate.cpp:220-225):
-//   ct   uint8   cell type (values of src/CAtypes.hpp:11) | SEED_BIT for cell-centred octahedra
-//   gid  int32   GrainID
-//   cts  int32   CritTimeStep (liquidus crossing step)
-//   lid  int8    LayerID
-//   dl   float   octahedron half-diagonal (DiagonalLength)
-//   thr  float   lower bound on the critical lengths of the still-liquid neighbours (0: unknown)
-//   cx/cy/cz double  octahedron centre as computed at capture (DOCenter is its float rounding)
+// Cell-state layout on the GPU: one 32-byte record per cell of the whole local domain, in the
+// reference linear order p = z*X*Y + x*Y + y with y fastest (src/CAupdate.cpp:220-225). Active
+// cells are sparse in that order (a front of a few percent of each plane over a band of planes),
+// so everything a step reads or atomically updates for a cell sits in one aligned 32-byte sector:
 //   lm   uint64  liquid neighbours of an active cell: bits 0-25 now, bits 32-57 as they were at
 //                this step's claim; bit 31 set: not yet known (a new cell, or never active)
+//   dl   float   octahedron half-diagonal (DiagonalLength)
+//   thr  float   lower bound on the critical lengths of the still-liquid neighbours (0: unknown)
+//   cts  int32   CritTimeStep (liquidus crossing step)
+//   gid  int32   GrainID
 //   win  uint32  lowest index of a cell that claimed this liquid cell in the current step
 //                (0xFFFFFFFF: unclaimed). A claimed cell is captured in the same step and never
 //                turns liquid again, so a claim word is read in the step that wrote it only.
+//   lid  int8    LayerID
+//   ct   uint8   cell type (values of src/CAtypes.hpp:11) | kSeedBit for cell-centred octahedra
+// The octahedron centre (double, as computed at capture; DOCenter is its float rounding) is a
+// second 32-byte record per cell, read only when a cell evaluates critical lengths or captures.
 // The reference's 26 critical diagonal lengths per cell (104 B) are not stored: they are
 // recomputed bit-exactly from the double centre (captured cells) or read from a per-orientation
 // table (cell-centred octahedra: nuclei and the initial substrate).
@@ -97,6 +100,8 @@ struct StepStats {
     unsigned long long nucleated;       // nucleation events counted as the reference's nn
     unsigned long long waiting;         // frontier entries still above their liquidus time
     unsigned long long activated;       // all nucleation events (interior or not)
+    unsigned long long evaluated;       // frontier cells whose critical lengths were evaluated
+    unsigned long long masks;           // liquid-neighbour masks built from the neighbour types
 };
 
 // Termination-check reduction (src/CAupdate.cpp:738-758 plus the min CritTimeStep of :792-808).
@@ -105,20 +110,41 @@ struct CheckResult {
     unsigned long long next_cts;   // min CritTimeStep (as unsigned long) of liquid cells in the layer
 };
 
+struct alignas(32) Cell {
+    unsigned long long lm;
+    float dl;
+    float thr;
+    int32_t cts;
+    int32_t gid;
+    uint32_t win;
+    int8_t lid;
+    uint8_t ct;
+    uint16_t pad;
+};
+static_assert(sizeof(Cell) == 32, "one sector per cell");
+struct alignas(32) Centre {
+    double x, y, z, pad;
+};
+static_assert(sizeof(Centre) == 32, "one sector per cell");
+
+// Per-cell fields a caller can read back (values of EXACA_FIELD_* in the C interface).
+enum FieldKind : int {
+    kFieldCellType = 0,
+    kFieldGrainId = 1,
+    kFieldDiagonal = 2,
+    kFieldCentre = 3,
+    kFieldCritTimeStep = 4,
+    kFieldLayerId = 5,
+    kFieldCellTypeRaw = 6,
+};
+
 // Everything a kernel needs to address one CA domain.
 struct DomainView {
     int X, Y, nz, nzA;      // MyXSlices, MyYSlices, nz, nzActive (ZBound_Low = 0 on one GPU)
     int xoff, yoff;         // MyXOffset, MyYOffset (-1 for one rank)
     long long ncell;        // X*Y*nz
-    uint8_t *ct;
-    int32_t *gid;
-    int32_t *cts;
-    int8_t *lid;
-    float *dl;
-    float *thr;
-    double *cx, *cy, *cz;
-    unsigned long long *lm;
-    uint32_t *win;
+    Cell *cells;
+    Centre *centre;
     uint32_t *list[2];      // frontier of a step of parity p (list[1-p] also holds the step's
                             // solidification candidates until the next frontier is written)
     uint32_t *claim_cell;   // cells with claims in the current step, stored in their own frontier

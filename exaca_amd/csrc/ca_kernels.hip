@@ -279,6 +279,28 @@ __device__ __forceinline__ uint32_t wave_append(bool want, uint32_t *counter) {
     return base + static_cast<uint32_t>(__popcll(b & ((1ull << lane) - 1)));
 }
 
+// A cell record read as two 16-byte loads (one 32-byte sector).
+struct CellWords {
+    unsigned long long lm;
+    float dl, thr;
+    int32_t cts, gid;
+    int8_t lid;
+    uint8_t ct;
+};
+__device__ __forceinline__ CellWords load_cell(const Cell *c) {
+    const uint4 a = reinterpret_cast<const uint4 *>(c)[0];
+    const uint4 b = reinterpret_cast<const uint4 *>(c)[1];
+    CellWords w;
+    w.lm = static_cast<unsigned long long>(a.x) | (static_cast<unsigned long long>(a.y) << 32);
+    w.dl = __uint_as_float(a.z);
+    w.thr = __uint_as_float(a.w);
+    w.cts = static_cast<int32_t>(b.x);
+    w.gid = static_cast<int32_t>(b.y);
+    w.lid = static_cast<int8_t>(b.w & 0xFFu);
+    w.ct = static_cast<uint8_t>((b.w >> 8) & 0xFFu);
+    return w;
+}
+
 __device__ __forceinline__ void set_active_bit(const DomainView &d, uint32_t q) {
     atomicOr(&d.bits[q >> 5], 1u << (q & 31));
 }
@@ -301,23 +323,22 @@ __device__ __forceinline__ void nucleate_body(const DomainView &d, int p, int la
     unsigned long long nn = 0, act = 0;
     for (int i = b * kBlock + threadIdx.x; i < d.nnuclei; i += nb * kBlock) {
         const uint32_t q = static_cast<uint32_t>(d.nloc[i]);
-        if (!(cycle >= d.ntime[i] && d.ct[q] == kLiquid && d.lid[q] <= layer)) continue;
+        if (!(cycle >= d.ntime[i] && d.cells[q].ct == kLiquid && d.cells[q].lid <= layer)) continue;
         const Grid gr = grid_of(d);
         const Coord c = decode(q, gr);
         if (c.x > 0 && c.x < d.X - 1 && c.y > 0 && c.y < d.Y - 1) nn++;
-        d.dl[q] = 0.01f;
-        d.thr[q] = 0.0f;
-        d.cx[q] = (c.x + d.xoff) + 0.5;
-        d.cy[q] = (c.y + d.yoff) + 0.5;
-        d.cz[q] = c.z + 0.5;
-        d.lm[q] = kLmUnknown;
-        d.ct[q] = kActive | kSeedBit;
+        d.cells[q].dl = 0.01f;
+        d.cells[q].thr = 0.0f;
+        d.centre[q] = Centre{(c.x + d.xoff) + 0.5, (c.y + d.yoff) + 0.5, c.z + 0.5, 0.0};
+        d.cells[q].lm = kLmUnknown;
+        d.cells[q].ct = kActive | kSeedBit;
         set_active_bit(d, q);
         // q is no longer liquid for any neighbour
         const uint32_t valid = valid_neighbours(c, gr.X, gr.Y, gr.nzA);
 #pragma unroll
         for (int l = 0; l < 26; l++)
-            if ((valid >> l) & 1u) atomicAnd(&d.lm[q + gr.off[l]], ~(1ull << kOpposite.v[l]));
+            if ((valid >> l) & 1u)
+                atomicAnd(reinterpret_cast<uint32_t *>(&d.cells[q + gr.off[l]].lm), ~(1u << kOpposite.v[l]));
         atomicMin(&ctl->rlo[p], q);
         atomicMax(&ctl->rhi[p], q);
         act++;
@@ -336,19 +357,108 @@ __device__ __forceinline__ void nucleate_body(const DomainView &d, int p, int la
 // the liquid state; a cell whose mask is not yet known (new this step) reads the 26 neighbour
 // types once. A cell past its liquidus time with no liquid neighbour turns solid here (no capture
 // of this step can give it one).
-// thr[j] is a lower bound on the critical length of every neighbour that is still liquid: the
+// thr is a lower bound on the critical length of every neighbour that is still liquid: the
 // smallest critical length (or a proven lower bound of it) over the liquid neighbours the cell did
 // not claim when it last evaluated them. Liquid sets only shrink and a cell's octahedron centre is
 // fixed, so while the half-diagonal stays below thr no neighbour can be captured and the step
 // needs neither the octahedron nor any critical length. A new active cell starts at thr = 0.
 // Cells with claims are appended to the claim lists inside the workgroup's own frontier slice (an
 // LDS counter, no global atomics), and the slice's count goes to slice_claims[b] for pass B.
+//
+// The common case (mask known, half-diagonal below thr) reads one record and writes the grown
+// half-diagonal. The rare cases, a mask to build or critical lengths to evaluate (a few percent of
+// the cells but present in nearly every wave), are queued in LDS and run afterwards with every lane
+// busy, instead of holding their whole wave at one or two active lanes.
+
+constexpr uint32_t kQueueCap = 4 * kBlock;
+constexpr uint32_t kItemNewMask = 0xFFFFFFFFu;  // queue tag: mask not yet known (not a float bit pattern used)
+
+struct ClaimCounts {
+    unsigned long long front, nwait, nsol, neval, nmask;
+};
+
+// Critical lengths of the liquid neighbours of frontier cell j (half-diagonal dlj): claims on
+// those reached; thr becomes the least length (or bound) of the others. Returns the claimed mask.
+__device__ uint32_t evaluate_claims(const DomainView &d, const Grid &gr, uint32_t j, float dlj, uint32_t liquid,
+                                    int32_t gid, uint8_t ct) {
+    int o = orientation_of(gid);
+    if (o < 0) {
+        atomicOr(&d.ctl->error, 2);
+        o = 0;
+    }
+    uint32_t claimed = 0;
+    float next = __builtin_inff();
+    if (ct & kSeedBit) {
+        const float *tab = d.seed_cdl + 26 * o;
+        for (uint32_t m = liquid; m; m &= m - 1) {
+            const int l = __builtin_ctz(m);
+            const float cdl = tab[l];
+            if (dlj >= cdl) {
+                atomicMin(&d.cells[j + gr.offset(l)].win, j);
+                claimed |= 1u << l;
+            }
+            else {
+                next = fminf(next, cdl);
+            }
+        }
+    }
+    else {
+        const Coord c = decode(j, gr);
+        const Centre e = d.centre[j];
+        const Faces f = faces_of(d.guv + 9 * o);
+        const double xp = (c.x + d.xoff) + 0.5, yp = (c.y + d.yoff) + 0.5, zp = c.z + 0.5;
+        // The critical length of a neighbour at distance |e| from the centre is at least
+        // sqrt(reach)|e| (reach from the orientation's deviation from orthonormality, with a
+        // margin far above rounding): below that the exact length is not needed, and the bound,
+        // rounded down to float, stands in for it in thr.
+        const double reach = d.reach[o];
+        const double dl2 = static_cast<double>(dlj) * static_cast<double>(dlj);
+        for (uint32_t m = liquid; m; m &= m - 1) {
+            const int l = __builtin_ctz(m);
+            const double ex = xp + nb_dx(l) - e.x, ey = yp + nb_dy(l) - e.y, ez = zp + nb_dz(l) - e.z;
+            const double r2 = reach * (ex * ex + ey * ey + ez * ez);
+            if (dl2 < r2) {
+                next = fminf(next, __double2float_rd(sqrt(r2)));
+                continue;
+            }
+            const float cdl = critical_length(f, xp, yp, zp, e.x, e.y, e.z, l);
+            if (dlj >= cdl) {
+                atomicMin(&d.cells[j + gr.offset(l)].win, j);
+                claimed |= 1u << l;
+            }
+            else {
+                next = fminf(next, cdl);
+            }
+        }
+    }
+    d.cells[j].thr = next;
+    return claimed;
+}
+
+// The half-diagonal after this step's growth (src/CAupdate.cpp:257-261), stored.
+__device__ __forceinline__ float grow_cell(const DomainView &d, uint32_t j, int t, int cycle, int njumps, float dl0) {
+    int k = executed_since(t, cycle, d.ctl, njumps);
+    k = k < 0 ? 0 : (k < d.ngrow - 1 ? k : d.ngrow - 1);
+    const float dlj = static_cast<float>(static_cast<double>(dl0) + d.grow[k]);
+    d.cells[j].dl = dlj;
+    return dlj;
+}
+
+__device__ __forceinline__ void solidify(const DomainView &d, uint32_t j) {
+    d.cells[j].ct = kSolid;
+    clear_active_bit(d, j);
+}
+
 __device__ __forceinline__ void claim_body(const DomainView &d, int p, int layer, uint32_t b, uint32_t nb) {
     __shared__ unsigned long long sh[kBlock / 64];
-    __shared__ uint32_t nclaim;
+    __shared__ uint32_t nclaim, nq;
+    __shared__ uint32_t q_cell[kQueueCap], q_tag[kQueueCap], q_liq[kQueueCap];
     StepCtl *ctl = d.ctl;
     if (ctl->done) return;
-    if (threadIdx.x == 0) nclaim = 0;
+    if (threadIdx.x == 0) {
+        nclaim = 0;
+        nq = 0;
+    }
     __syncthreads();
     const int cycle = ctl->cycle[p];
     const int njumps = ctl->njumps;
@@ -356,208 +466,262 @@ __device__ __forceinline__ void claim_body(const DomainView &d, int p, int layer
     frontier_slice(ctl->count[p], d.list_cap, b, nb, begin, end);
     const uint32_t *list = d.list[p];
     const Grid gr = grid_of(d);
-    unsigned long long front = 0, nwait = 0, nsol = 0;
-    for (uint32_t i0 = begin; i0 < end; i0 += kBlock) {
-        const uint32_t i = i0 + threadIdx.x;
-        uint32_t j = 0, claimed = 0;
-        if (i < end) {
-            j = list[i];
-            const int t = d.cts[j];
-            const int8_t lay = d.lid[j];
-            const float dl0 = d.dl[j];
-            const float thr = d.thr[j];
-            unsigned long long w = d.lm[j];
-            if (w & kLmUnknown) {
-                const Coord c = decode(j, gr);
-                const uint32_t valid = valid_neighbours(c, gr.X, gr.Y, gr.nzA);
-                unsigned long long liq = 0;
+    ClaimCounts n{0, 0, 0, 0, 0};
+
+    // Runs the queued cells, one per lane, and appends their claims.
+    auto drain = [&]() {
+        const uint32_t total = nq;
+        for (uint32_t k0 = 0; k0 < total; k0 += kBlock) {
+            const uint32_t k = k0 + threadIdx.x;
+            uint32_t j = 0, claimed = 0;
+            if (k < total) {
+                j = q_cell[k];
+                const uint32_t tag = q_tag[k];
+                const CellWords cw = load_cell(d.cells + j);
+                if (tag == kItemNewMask) {
+                    const Coord c = decode(j, gr);
+                    const uint32_t valid = valid_neighbours(c, gr.X, gr.Y, gr.nzA);
+                    unsigned long long liq = 0;
 #pragma unroll
-                for (int l = 0; l < 26; l++) {
-                    const uint32_t q = ((valid >> l) & 1u) ? j + gr.off[l] : j;
-                    liq |= static_cast<unsigned long long>(d.ct[q] == kLiquid) << l;
-                }
-                w = liq | (liq << 32);
-                d.lm[j] = w;
-            }
-            else if ((w & kLmNow) != (w >> 32)) {
-                w = (w & kLmNow) | ((w & kLmNow) << 32);
-                d.lm[j] = w;
-            }
-            const uint32_t liquid = static_cast<uint32_t>(w & kLmNow);
-            if (!(cycle > t && lay <= layer)) {
-                nwait++;
-            }
-            else {
-                front++;
-                int k = executed_since(t, cycle, ctl, njumps);
-                k = k < 0 ? 0 : (k < d.ngrow - 1 ? k : d.ngrow - 1);
-                const float dlj = static_cast<float>(static_cast<double>(dl0) + d.grow[k]);
-                d.dl[j] = dlj;
-                if (!liquid) {
-                    d.ct[j] = kSolid;
-                    clear_active_bit(d, j);
-                    nsol++;
-                }
-                else if (dlj >= thr) {
-                    int o = orientation_of(d.gid[j]);
-                    if (o < 0) {
-                        atomicOr(&ctl->error, 2);
-                        o = 0;
+                    for (int l = 0; l < 26; l++) {
+                        const uint32_t q = ((valid >> l) & 1u) ? j + gr.off[l] : j;
+                        liq |= static_cast<unsigned long long>(d.cells[q].ct == kLiquid) << l;
                     }
-                    float next = __builtin_inff();
-                    if (d.ct[j] & kSeedBit) {
-                        const float *tab = d.seed_cdl + 26 * o;
-                        for (uint32_t m = liquid; m; m &= m - 1) {
-                            const int l = __builtin_ctz(m);
-                            const float cdl = tab[l];
-                            if (dlj >= cdl) {
-                                atomicMin(&d.win[j + gr.offset(l)], j);
-                                claimed |= 1u << l;
-                            }
-                            else {
-                                next = fminf(next, cdl);
-                            }
-                        }
+                    d.cells[j].lm = liq | (liq << 32);
+                    n.nmask++;
+                    const uint32_t liquid = static_cast<uint32_t>(liq);
+                    if (!(cycle > cw.cts && cw.lid <= layer)) {
+                        n.nwait++;
                     }
                     else {
-                        const Coord c = decode(j, gr);
-                        const double cx = d.cx[j], cy = d.cy[j], cz = d.cz[j];
-                        const Faces f = faces_of(d.guv + 9 * o);
-                        const double xp = (c.x + d.xoff) + 0.5, yp = (c.y + d.yoff) + 0.5, zp = c.z + 0.5;
-                        // The critical length of a neighbour at distance |e| from the centre is at
-                        // least sqrt(reach)|e| (reach from the orientation's deviation from
-                        // orthonormality, with a margin far above rounding): below that the exact
-                        // length is not needed, and the bound, rounded down to float, stands in
-                        // for it in thr.
-                        const double reach = d.reach[o];
-                        const double dl2 = static_cast<double>(dlj) * static_cast<double>(dlj);
-                        for (uint32_t m = liquid; m; m &= m - 1) {
-                            const int l = __builtin_ctz(m);
-                            const double ex = xp + nb_dx(l) - cx, ey = yp + nb_dy(l) - cy, ez = zp + nb_dz(l) - cz;
-                            const double r2 = reach * (ex * ex + ey * ey + ez * ez);
-                            if (dl2 < r2) {
-                                next = fminf(next, __double2float_rd(sqrt(r2)));
-                                continue;
-                            }
-                            const float cdl = critical_length(f, xp, yp, zp, cx, cy, cz, l);
-                            if (dlj >= cdl) {
-                                atomicMin(&d.win[j + gr.offset(l)], j);
-                                claimed |= 1u << l;
-                            }
-                            else {
-                                next = fminf(next, cdl);
-                            }
+                        n.front++;
+                        const float dlj = grow_cell(d, j, cw.cts, cycle, njumps, cw.dl);
+                        if (!liquid) {
+                            solidify(d, j);
+                            n.nsol++;
+                        }
+                        else if (dlj >= cw.thr) {
+                            claimed = evaluate_claims(d, gr, j, dlj, liquid, cw.gid, cw.ct);
+                            n.neval++;
                         }
                     }
-                    d.thr[j] = next;
+                }
+                else {
+                    claimed = evaluate_claims(d, gr, j, __uint_as_float(tag), q_liq[k], cw.gid, cw.ct);
+                    n.neval++;
+                }
+            }
+            const uint32_t slot = begin + wave_append(claimed != 0, &nclaim);
+            if (claimed) {
+                d.claim_cell[slot] = j;
+                d.claim_bits[slot] = claimed;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) nq = 0;
+        __syncthreads();
+    };
+
+    for (uint32_t i0 = begin; i0 < end; i0 += kBlock) {
+        const uint32_t i = i0 + threadIdx.x;
+        bool defer = false;
+        uint32_t j = 0, tag = 0, liquid = 0;
+        if (i < end) {
+            j = list[i];
+            const CellWords cw = load_cell(d.cells + j);
+            if (cw.lm & kLmUnknown) {
+                defer = true;
+                tag = kItemNewMask;
+            }
+            else {
+                liquid = static_cast<uint32_t>(cw.lm & kLmNow);
+                if (liquid != static_cast<uint32_t>(cw.lm >> 32))
+                    d.cells[j].lm = static_cast<unsigned long long>(liquid) | (static_cast<unsigned long long>(liquid) << 32);
+                if (!(cycle > cw.cts && cw.lid <= layer)) {
+                    n.nwait++;
+                }
+                else {
+                    n.front++;
+                    const float dlj = grow_cell(d, j, cw.cts, cycle, njumps, cw.dl);
+                    if (!liquid) {
+                        solidify(d, j);
+                        n.nsol++;
+                    }
+                    else if (dlj >= cw.thr) {
+                        defer = true;
+                        tag = __float_as_uint(dlj);
+                    }
                 }
             }
         }
-        const uint32_t slot = begin + wave_append(claimed != 0, &nclaim);
-        if (claimed) {
-            d.claim_cell[slot] = j;
-            d.claim_bits[slot] = claimed;
+        const uint32_t qs = wave_append(defer, &nq);
+        if (defer) {
+            q_cell[qs] = j;
+            q_tag[qs] = tag;
+            q_liq[qs] = liquid;
         }
+        __syncthreads();
+        if (nq > kQueueCap - kBlock) drain();
     }
     __syncthreads();
+    if (nq) drain();
+    __syncthreads();
     if (threadIdx.x == 0) d.slice_claims[b] = nclaim;
-    unsigned long long t = block_sum(front, sh);
+    unsigned long long t = block_sum(n.front, sh);
     if (threadIdx.x == 0 && t) atomicAdd(&d.stats->frontier, t);
-    t = block_sum(nwait, sh);
+    t = block_sum(n.nwait, sh);
     if (threadIdx.x == 0 && t) atomicAdd(&d.stats->waiting, t);
-    t = block_sum(nsol, sh);
+    t = block_sum(n.nsol, sh);
     if (threadIdx.x == 0 && t) atomicAdd(&d.stats->solidified, t);
+    t = block_sum(n.neval, sh);
+    if (threadIdx.x == 0 && t) atomicAdd(&d.stats->evaluated, t);
+    t = block_sum(n.nmask, sh);
+    if (threadIdx.x == 0 && t) atomicAdd(&d.stats->masks, t);
 }
 
-// Pass B over the cells with claims, workgroup b over the claims of slice b of pass A (the same
-// grid): a claim whose word holds the claimant's own index is won
-// (the lowest index that reached the cell), and the winner builds the captured cell's octahedron
-// (src/CAupdate.cpp:353-563, 717). The captured cell then leaves the liquid masks of its
-// neighbours; a neighbour whose mask this empties is a solidification candidate of this step,
-// appended (per wave) to the candidate list held in the other frontier buffer.
-__device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t b, uint32_t nb) {
+// Pass B over the cells with claims. Workgroup a takes the claims of the kApplyGroup slices
+// a*kApplyGroup .. of pass A's nbc workgroups (a slice holds a few dozen claims: grouping fills the
+// waves and keeps the whole grid resident). A claim whose word holds the claimant's own index is won
+// (the lowest index that reached the cell). The won captures of a round of claimants are numbered
+// by a block scan and then built one per lane: the winner's octahedron gives the captured cell's
+// (src/CAupdate.cpp:353-563, 717), and the captured cell leaves the liquid masks of its
+// neighbours. Only masks that still hold it are updated: the masks are read first, and a cell's
+// bit is cleared by its capture alone, so the read decides exactly. A neighbour whose mask this
+// empties is a solidification candidate of this step, appended (per wave) to the candidate list
+// held in the other frontier buffer.
+constexpr uint32_t kApplyGroup = 4;
+__host__ __device__ constexpr uint32_t apply_blocks(uint32_t claim_blocks) {
+    return (claim_blocks + kApplyGroup - 1) / kApplyGroup;
+}
+
+// The low word of a cell's liquid-neighbour mask: bits 0-25 and the not-yet-known bit.
+__device__ __forceinline__ uint32_t *mask_word(const DomainView &d, uint32_t q) {
+    return reinterpret_cast<uint32_t *>(&d.cells[q].lm);
+}
+constexpr uint32_t kMaskUnknown = static_cast<uint32_t>(kLmUnknown);
+constexpr uint32_t kMaskNow = static_cast<uint32_t>(kLmNow);
+
+__device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t a, uint32_t nbc) {
     __shared__ unsigned long long sh[kBlock / 64];
+    __shared__ uint32_t wave_tot[kBlock / 64];
+    __shared__ uint32_t pre[kBlock + 1];
+    __shared__ uint32_t won_cell[kBlock], won_bits[kBlock];
     StepCtl *ctl = d.ctl;
     if (ctl->done) return;
-    uint32_t begin, end;
-    frontier_slice(ctl->count[p], d.list_cap, b, nb, begin, end);
-    const uint32_t n = begin + d.slice_claims[b];
+    uint32_t beg[kApplyGroup], upto[kApplyGroup], n = 0;
+#pragma unroll
+    for (uint32_t g = 0; g < kApplyGroup; g++) {
+        const uint32_t b = a * kApplyGroup + g;
+        uint32_t e = 0, cnt = 0;
+        beg[g] = 0;
+        if (b < nbc) {
+            frontier_slice(ctl->count[p], d.list_cap, b, nbc, beg[g], e);
+            cnt = d.slice_claims[b];
+        }
+        n += cnt;
+        upto[g] = n;
+    }
     const Grid gr = grid_of(d);
     uint32_t *cand = d.list[1 - p];
-    const int lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     unsigned long long ncap = 0;
-    for (uint32_t i0 = begin; i0 < n; i0 += kBlock) {
-        const uint32_t i = i0 + threadIdx.x;
-        uint32_t j = 0, todo = 0;
-        if (i < n) {
+    for (uint32_t k0 = 0; k0 < n; k0 += kBlock) {
+        // Won claims of this round's claimants, and their block-wide exclusive prefix.
+        const uint32_t k = k0 + threadIdx.x;
+        uint32_t j = 0, won = 0;
+        if (k < n) {
+            uint32_t i = beg[0] + k;
+#pragma unroll
+            for (uint32_t g = 1; g < kApplyGroup; g++)
+                if (k >= upto[g - 1]) i = beg[g] + (k - upto[g - 1]);
             j = d.claim_cell[i];
             for (uint32_t m = d.claim_bits[i]; m; m &= m - 1) {
                 const int l = __builtin_ctz(m);
-                todo |= static_cast<uint32_t>(d.win[j + gr.offset(l)] == j) << l;
+                won |= static_cast<uint32_t>(d.cells[j + gr.offset(l)].win == j) << l;
             }
         }
-        Coord c{0, 0, 0};
-        int h = 0;
-        const float *g = d.guv;
-        double cxold = 0, cyold = 0, czold = 0;
-        if (todo) {
-            c = decode(j, gr);
-            h = d.gid[j];
-            int o = orientation_of(h);
-            if (o < 0) {
-                atomicOr(&ctl->error, 2);
-                o = 0;
-            }
-            g = d.guv + 9 * o;
-            cxold = static_cast<float>(d.cx[j]);
-            cyold = static_cast<float>(d.cy[j]);
-            czold = static_cast<float>(d.cz[j]);
+        const uint32_t cnt = __builtin_popcount(won);
+        uint32_t wtot;
+        const uint32_t incl = wave_inclusive_scan(cnt, wtot);
+        if (lane == 63) wave_tot[wave] = wtot;
+        won_cell[threadIdx.x] = j;
+        won_bits[threadIdx.x] = won;
+        __syncthreads();
+        uint32_t base = 0, total = 0;
+        for (int w = 0; w < kBlock / 64; w++) {
+            if (w < wave) base += wave_tot[w];
+            total += wave_tot[w];
         }
-        // One capture per lane per round; the rounds are wave-uniform so that the candidate
-        // pushes of a round share one atomic.
-        while (__any(todo != 0)) {
+        pre[threadIdx.x] = base + incl - cnt;
+        if (threadIdx.x == 0) pre[kBlock] = total;
+        __syncthreads();
+        // One capture per lane: item t belongs to the claimant s with pre[s] <= t < pre[s + 1].
+        for (uint32_t t0 = 0; t0 < total; t0 += kBlock) {
+            const uint32_t t = t0 + threadIdx.x;
             uint32_t q = 0, emptied = 0;
-            if (todo) {
-                const int l = __builtin_ctz(todo);
-                todo &= todo - 1;
-                q = j + gr.offset(l);
+            if (t < total) {
+                uint32_t lo = 0, hi = kBlock;  // pre[lo] <= t < pre[hi]
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (pre[mid] <= t)
+                        lo = mid;
+                    else
+                        hi = mid;
+                }
+                const uint32_t jc = won_cell[lo];
+                uint32_t bits = won_bits[lo];
+                for (uint32_t r = t - pre[lo]; r; r--) bits &= bits - 1;
+                const int l = __builtin_ctz(bits);
+                q = jc + gr.offset(l);
+                const Coord c = decode(jc, gr);
+                const int h = d.cells[jc].gid;
+                int o = orientation_of(h);
+                if (o < 0) {
+                    atomicOr(&ctl->error, 2);
+                    o = 0;
+                }
+                const Centre e = d.centre[jc];
                 const double xp = (c.x + d.xoff + nb_dx(l)) + 0.5;
                 const double yp = (c.y + d.yoff + nb_dy(l)) + 0.5;
                 const double zp = (c.z + nb_dz(l)) + 0.5;
-                const Octahedron oc = new_octahedron(g, cxold, cyold, czold, xp, yp, zp);
-                d.gid[q] = h;
-                d.dl[q] = static_cast<float>(oc.dl);
-                d.thr[q] = 0.0f;
-                d.cx[q] = oc.cx;
-                d.cy[q] = oc.cy;
-                d.cz[q] = oc.cz;
-                d.lm[q] = kLmUnknown;
-                d.ct[q] = kActive;
+                const Octahedron oc = new_octahedron(d.guv + 9 * o, static_cast<float>(e.x), static_cast<float>(e.y),
+                                                     static_cast<float>(e.z), xp, yp, zp);
+                d.cells[q].gid = h;
+                d.cells[q].dl = static_cast<float>(oc.dl);
+                d.cells[q].thr = 0.0f;
+                d.centre[q] = Centre{oc.cx, oc.cy, oc.cz, 0.0};
+                d.cells[q].lm = kLmUnknown;
+                d.cells[q].ct = kActive;
                 set_active_bit(d, q);
                 ncap++;
                 const Coord cq{c.x + nb_dx(l), c.y + nb_dy(l), c.z + nb_dz(l)};
                 const uint32_t valid = valid_neighbours(cq, gr.X, gr.Y, gr.nzA);
+                uint32_t mw[26];
+#pragma unroll
+                for (int k = 0; k < 26; k++) mw[k] = ((valid >> k) & 1u) ? *mask_word(d, q + gr.off[k]) : kMaskUnknown;
 #pragma unroll
                 for (int k = 0; k < 26; k++) {
-                    if ((valid >> k) & 1u) {
-                        const unsigned long long bit = 1ull << kOpposite.v[k];
-                        const unsigned long long old = atomicAnd(&d.lm[q + gr.off[k]], ~bit);
-                        emptied |= static_cast<uint32_t>((old & (kLmUnknown | kLmNow)) == bit) << k;
+                    const uint32_t bit = 1u << kOpposite.v[k];
+                    if ((mw[k] & (kMaskUnknown | bit)) == bit) {
+                        const uint32_t old = atomicAnd(mask_word(d, q + gr.off[k]), ~bit);
+                        emptied |= static_cast<uint32_t>((old & (kMaskUnknown | kMaskNow)) == bit) << k;
                     }
                 }
             }
-            const uint32_t cnt = __builtin_popcount(emptied);
-            uint32_t tot;
-            const uint32_t incl = wave_inclusive_scan(cnt, tot);
-            if (tot) {
-                uint32_t base = 0;
-                if (lane == 63) base = atomicAdd(&ctl->ncand, tot);
-                base = __shfl(base, 63, 64);
-                uint32_t pos = base + incl - cnt;
+            // Candidates: one scan and one atomic per wave.
+            const uint32_t ce = __builtin_popcount(emptied);
+            uint32_t ctot;
+            const uint32_t cincl = wave_inclusive_scan(ce, ctot);
+            if (ctot) {
+                uint32_t cbase = 0;
+                if (lane == 63) cbase = atomicAdd(&ctl->ncand, ctot);
+                cbase = __shfl(cbase, 63, 64);
+                uint32_t pos = cbase + cincl - ce;
                 for (uint32_t r = emptied; r; r &= r - 1, pos++)
                     if (pos < d.list_cap) cand[pos] = q + gr.offset(__builtin_ctz(r));
             }
         }
+        __syncthreads();
     }
     const unsigned long long t = block_sum(ncap, sh);
     if (threadIdx.x == 0 && t) atomicAdd(&d.stats->captures, t);
@@ -578,16 +742,16 @@ __device__ __forceinline__ void settle_body(const DomainView &d, int p, int laye
     unsigned long long nsol = 0;
     for (uint32_t i = b * kBlock + threadIdx.x; i < n; i += nb * kBlock) {
         const uint32_t r = cand[i];
-        if ((d.ct[r] & kTypeMask) != kActive) continue;
-        if (!(cycle > d.cts[r] && d.lid[r] <= layer)) continue;
+        if ((d.cells[r].ct & kTypeMask) != kActive) continue;
+        if (!(cycle > d.cells[r].cts && d.cells[r].lid <= layer)) continue;
         bool solid = true;
-        for (uint32_t m = static_cast<uint32_t>(d.lm[r] >> 32); m; m &= m - 1)
-            if (d.win[r + gr.offset(__builtin_ctz(m))] >= r) {
+        for (uint32_t m = static_cast<uint32_t>(d.cells[r].lm >> 32); m; m &= m - 1)
+            if (d.cells[r + gr.offset(__builtin_ctz(m))].win >= r) {
                 solid = false;
                 break;
             }
         if (solid) {
-            d.ct[r] = kSolid;
+            d.cells[r].ct = kSolid;
             clear_active_bit(d, r);
             nsol++;
         }
@@ -739,7 +903,9 @@ __global__ __launch_bounds__(kBlock) void k_nucleate(DomainView d, int p, int la
 __global__ __launch_bounds__(kBlock) void k_claim(DomainView d, int p, int layer) {
     claim_body(d, p, layer, blockIdx.x, gridDim.x);
 }
-__global__ __launch_bounds__(kBlock) void k_apply(DomainView d, int p) { apply_body(d, p, blockIdx.x, gridDim.x); }
+__global__ __launch_bounds__(kBlock) void k_apply(DomainView d, int p, int nbc) {
+    apply_body(d, p, blockIdx.x, static_cast<uint32_t>(nbc));
+}
 __global__ __launch_bounds__(kBlock) void k_settle(DomainView d, int p, int layer) {
     settle_body(d, p, layer, blockIdx.x, gridDim.x);
 }
@@ -761,9 +927,9 @@ __global__ __launch_bounds__(kBlock) void k_claim_n(const DomainView *v, int p, 
     const DomainView &d = v[blockIdx.y];
     if (d.ctl) claim_body(d, p, layer, blockIdx.x, gridDim.x);
 }
-__global__ __launch_bounds__(kBlock) void k_apply_n(const DomainView *v, int p) {
+__global__ __launch_bounds__(kBlock) void k_apply_n(const DomainView *v, int p, int nbc) {
     const DomainView &d = v[blockIdx.y];
-    if (d.ctl) apply_body(d, p, blockIdx.x, gridDim.x);
+    if (d.ctl) apply_body(d, p, blockIdx.x, static_cast<uint32_t>(nbc));
 }
 __global__ __launch_bounds__(kBlock) void k_settle_n(const DomainView *v, int p, int layer) {
     const DomainView &d = v[blockIdx.y];
@@ -790,10 +956,10 @@ __global__ __launch_bounds__(kBlock) void k_check(DomainView d, int cycle, int l
     const long long active_end = static_cast<long long>(d.X) * d.Y * d.nzA;
     for (long long q = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x; q < d.ncell;
          q += static_cast<long long>(gridDim.x) * kBlock) {
-        const int t = d.ct[q] & kTypeMask;
-        if (d.lid[q] == layer) {
+        const int t = d.cells[q].ct & kTypeMask;
+        if (d.cells[q].lid == layer) {
             if (t == kLiquid) {
-                const int crit = d.cts[q];
+                const int crit = d.cells[q].cts;
                 if (crit > cycle)
                     c[0]++;
                 else
@@ -832,10 +998,47 @@ __global__ __launch_bounds__(kBlock) void k_seed_init(DomainView d, const uint32
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
         const uint32_t q = cells[i];
         const Coord c = decode(q, d.X, d.Y);
-        d.dl[q] = 0.01f;
-        d.cx[q] = (c.x + d.xoff) + 0.5;
-        d.cy[q] = (c.y + d.yoff) + 0.5;
-        d.cz[q] = c.z + 0.5;
+        d.cells[q].dl = 0.01f;
+        d.centre[q] = Centre{(c.x + d.xoff) + 0.5, (c.y + d.yoff) + 0.5, c.z + 0.5, 0.0};
+    }
+}
+
+// Initial cell records from the host's compact arrays: no octahedron, no claim, liquid mask not
+// yet known; the centres start at zero.
+__global__ __launch_bounds__(kBlock) void k_pack(DomainView d, const uint8_t *ct, const int8_t *lid,
+                                                 const int32_t *gid, const int32_t *cts) {
+    for (long long q = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x; q < d.ncell;
+         q += static_cast<long long>(gridDim.x) * kBlock) {
+        uint4 *c = reinterpret_cast<uint4 *>(d.cells + q);
+        c[0] = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u);
+        c[1] = make_uint4(static_cast<uint32_t>(cts[q]), static_cast<uint32_t>(gid[q]), kClaimNone,
+                          static_cast<uint32_t>(static_cast<uint8_t>(lid[q])) | (static_cast<uint32_t>(ct[q]) << 8));
+        d.centre[q] = Centre{0.0, 0.0, 0.0, 0.0};
+    }
+}
+
+// One per-cell field in the caller's layout (int32 per cell unless noted).
+__global__ __launch_bounds__(kBlock) void k_extract(DomainView d, int field, void *out) {
+    for (long long q = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x; q < d.ncell;
+         q += static_cast<long long>(gridDim.x) * kBlock) {
+        const Cell &c = d.cells[q];
+        switch (field) {
+        case kFieldCellType: static_cast<int32_t *>(out)[q] = c.ct & kTypeMask; break;
+        case kFieldCellTypeRaw: static_cast<uint8_t *>(out)[q] = c.ct; break;  // uint8
+        case kFieldGrainId: static_cast<int32_t *>(out)[q] = c.gid; break;
+        case kFieldDiagonal: static_cast<float *>(out)[q] = c.dl; break;  // float
+        case kFieldCritTimeStep: static_cast<int32_t *>(out)[q] = c.cts; break;
+        case kFieldLayerId: static_cast<int32_t *>(out)[q] = c.lid; break;
+        case kFieldCentre: {  // float x, y, z
+            const Centre &e = d.centre[q];
+            float *o = static_cast<float *>(out) + 3 * q;
+            o[0] = static_cast<float>(e.x);
+            o[1] = static_cast<float>(e.y);
+            o[2] = static_cast<float>(e.z);
+            break;
+        }
+        default: break;
+        }
     }
 }
 
@@ -878,7 +1081,7 @@ void launch_claim(const DomainView &d, int parity, int layer, int blocks, hipStr
 }
 void launch_apply(const DomainView &d, int parity, int layer, int blocks, hipStream_t s) {
     const int cb = static_cast<int>(d.nchunks < static_cast<uint32_t>(blocks) ? d.nchunks : blocks);
-    k_apply<<<blocks, kBlock, 0, s>>>(d, parity);
+    k_apply<<<apply_blocks(blocks), kBlock, 0, s>>>(d, parity, blocks);
     k_settle<<<blocks, kBlock, 0, s>>>(d, parity, layer);
     k_bits_count<<<cb, kBlock, 0, s>>>(d, parity);
     k_bits_scan<<<1, kBlock, 0, s>>>(d, parity);
@@ -895,7 +1098,7 @@ void launch_step_timed(const DomainView &d, int parity, int layer, int blocks, h
     k_claim<<<blocks, kBlock, 0, s>>>(d, parity, layer);
     rec();
     rec();
-    k_apply<<<blocks, kBlock, 0, s>>>(d, parity);
+    k_apply<<<apply_blocks(blocks), kBlock, 0, s>>>(d, parity, blocks);
     rec();
     rec();
     k_settle<<<blocks, kBlock, 0, s>>>(d, parity, layer);
@@ -914,7 +1117,7 @@ void launch_step_n(const DomainView *views, int slots, int parity, int layer, in
                    hipStream_t s) {
     k_nucleate_n<<<dim3(nuc_blocks, slots), kBlock, 0, s>>>(views, parity, layer);
     k_claim_n<<<dim3(blocks, slots), kBlock, 0, s>>>(views, parity, layer);
-    k_apply_n<<<dim3(blocks, slots), kBlock, 0, s>>>(views, parity);
+    k_apply_n<<<dim3(apply_blocks(blocks), slots), kBlock, 0, s>>>(views, parity, blocks);
     k_settle_n<<<dim3(blocks, slots), kBlock, 0, s>>>(views, parity, layer);
     k_bits_count_n<<<dim3(blocks, slots), kBlock, 0, s>>>(views, parity);
     k_bits_scan_n<<<dim3(1, slots), kBlock, 0, s>>>(views, parity);
@@ -925,6 +1128,13 @@ void launch_check(const DomainView &d, int cycle, int layer, CheckResult *out, h
 }
 void launch_seed_init(const DomainView &d, const uint32_t *cells, uint32_t n, hipStream_t s) {
     k_seed_init<<<grid_for(n, 1024), kBlock, 0, s>>>(d, cells, n);
+}
+void launch_pack(const DomainView &d, const uint8_t *ct, const int8_t *lid, const int32_t *gid, const int32_t *cts,
+                 hipStream_t s) {
+    k_pack<<<grid_for(d.ncell, 1 << 16), kBlock, 0, s>>>(d, ct, lid, gid, cts);
+}
+void launch_extract(const DomainView &d, int field, void *out, hipStream_t s) {
+    k_extract<<<grid_for(d.ncell, 1 << 16), kBlock, 0, s>>>(d, field, out);
 }
 void launch_copy(const uint4 *src, uint4 *dst, size_t n, hipStream_t s) {
     k_copy<<<grid_for(static_cast<long long>(n), 1 << 16), kBlock, 0, s>>>(src, dst, n);

@@ -26,6 +26,24 @@ void launch_seed_init(const DomainView &d, const uint32_t *cells, uint32_t n, hi
 // frontier rebuild (count, scan, write). ev holds 2 * kTimedKernels events.
 constexpr int kTimedKernels = 7;
 void launch_step_timed(const DomainView &d, int parity, int layer, int blocks, hipStream_t s, hipEvent_t *ev);
+// Initial cell records from compact device arrays (CellType with the seed bit, LayerID, GrainID,
+// CritTimeStep), and one field back out of the records (FieldKind layout).
+void launch_pack(const DomainView &d, const uint8_t *ct, const int8_t *lid, const int32_t *gid, const int32_t *cts,
+                 hipStream_t s);
+void launch_extract(const DomainView &d, int field, void *out, hipStream_t s);
+// Bytes per cell of a field as launch_extract writes it (0: not a per-cell field).
+inline size_t field_bytes(int field) {
+    switch (field) {
+    case kFieldCellTypeRaw: return 1;
+    case kFieldCentre: return 12;
+    case kFieldCellType:
+    case kFieldGrainId:
+    case kFieldDiagonal:
+    case kFieldCritTimeStep:
+    case kFieldLayerId: return 4;
+    default: return 0;
+    }
+}
 // Plain copy of n 16-byte words (HBM calibration).
 void launch_copy(const uint4 *src, uint4 *dst, size_t n, hipStream_t s);
 // Device-side self test of the capture geometry against host values: one octahedron per input row.

@@ -32,6 +32,55 @@ std::vector<uint32_t> active_bits(const HostDomain &hd) {
     for (uint32_t q : hd.active) b[q >> 5] |= 1u << (q & 31);
     return b;
 }
+
+// Initial cell records of a domain: the compact host arrays are staged in device buffers that are
+// free until the first step (the claim lists and the second frontier list: at least 12 bytes per
+// cell) and packed on the device. The initial frontier goes to list[0], with the cell-centred
+// octahedra of the initially active cells. Synchronises the stream (the host sources are
+// temporaries).
+void upload_cells(const DomainView &d, const HostDomain &hd, hipStream_t s) {
+    const size_t n = static_cast<size_t>(hd.ncell);
+    std::vector<uint8_t> ct = hd.ct;
+    for (uint32_t q : hd.active) ct[q] |= kSeedBit;
+    uint8_t *ct_d = reinterpret_cast<uint8_t *>(d.claim_cell);
+    int8_t *lid_d = reinterpret_cast<int8_t *>(ct_d + n);
+    int32_t *gid_d = reinterpret_cast<int32_t *>(d.list[1]);
+    int32_t *cts_d = reinterpret_cast<int32_t *>(d.claim_bits);
+    HIP_OK(hipMemcpyAsync(ct_d, ct.data(), n, hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemcpyAsync(lid_d, hd.lid.data(), n, hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemcpyAsync(gid_d, hd.gid.data(), 4 * n, hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemcpyAsync(cts_d, hd.cts.data(), 4 * n, hipMemcpyHostToDevice, s));
+    launch_pack(d, ct_d, lid_d, gid_d, cts_d, s);
+    HIP_OK(hipGetLastError());
+    const std::vector<uint32_t> bits = active_bits(hd);
+    HIP_OK(hipMemcpyAsync(d.bits, bits.data(), 4 * bits.size(), hipMemcpyHostToDevice, s));
+    if (!hd.active.empty()) {
+        HIP_OK(hipMemcpyAsync(d.list[0], hd.active.data(), 4 * hd.active.size(), hipMemcpyHostToDevice, s));
+        launch_seed_init(d, d.list[0], static_cast<uint32_t>(hd.active.size()), s);
+        HIP_OK(hipGetLastError());
+    }
+    HIP_OK(hipStreamSynchronize(s));
+}
+
+// One per-cell field of a domain into host memory (the layout launch_extract writes).
+void download_field(const DomainView &d, int field, void *dst, size_t dst_bytes, hipStream_t s) {
+    const size_t per = field_bytes(field);
+    if (per == 0) throw std::invalid_argument("unknown field " + std::to_string(field));
+    const size_t bytes = per * static_cast<size_t>(d.ncell);
+    if (dst_bytes < bytes) throw std::invalid_argument("destination too small");
+    void *tmp = nullptr;
+    HIP_OK(hipMalloc(&tmp, bytes));
+    try {
+        launch_extract(d, field, tmp, s);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpyAsync(dst, tmp, bytes, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+    } catch (...) {
+        (void)hipFree(tmp);
+        throw;
+    }
+    HIP_OK(hipFree(tmp));
+}
 }  // namespace exaca
 
 namespace {
@@ -105,20 +154,11 @@ void build_device_state(exaca_state *h) {
     d.yoff = hd.yoff;
     d.ncell = hd.ncell;
     const size_t n = static_cast<size_t>(hd.ncell);
-    d.ct = h->alloc<uint8_t>(n);
-    d.gid = h->alloc<int32_t>(n);
-    d.cts = h->alloc<int32_t>(n);
-    d.lid = h->alloc<int8_t>(n);
-    d.dl = h->alloc<float>(n);
-    d.thr = h->alloc<float>(n);
-    d.cx = h->alloc<double>(n);
-    d.cy = h->alloc<double>(n);
-    d.cz = h->alloc<double>(n);
-    d.win = h->alloc<uint32_t>(n);
+    d.cells = h->alloc<Cell>(n);
+    d.centre = h->alloc<Centre>(n);
     d.list_cap = static_cast<uint32_t>(n);
     d.list[0] = h->alloc<uint32_t>(n);
     d.list[1] = h->alloc<uint32_t>(n);
-    d.lm = h->alloc<unsigned long long>(n);
     d.claim_cell = h->alloc<uint32_t>(n);
     d.claim_bits = h->alloc<uint32_t>(n);
     d.slice_claims = h->alloc<uint32_t>(h->blocks);
@@ -152,23 +192,9 @@ void build_device_state(exaca_state *h) {
     d.ctl = h->ctl;
     d.stats = h->stats;
 
-    // Initial state: cell types with the seed bit on the initially active (cell-centred)
-    // octahedra; per-cell undercooling is not stored (it is a function of the steps executed
+    // Initial state; per-cell undercooling is not stored (it is a function of the steps executed
     // below the liquidus time).
-    std::vector<uint8_t> ct = hd.ct;
-    for (uint32_t q : hd.active) ct[q] |= kSeedBit;
     hipStream_t s = h->stream;
-    HIP_OK(hipMemcpyAsync(d.ct, ct.data(), n, hipMemcpyHostToDevice, s));
-    HIP_OK(hipMemcpyAsync(d.gid, hd.gid.data(), n * 4, hipMemcpyHostToDevice, s));
-    HIP_OK(hipMemcpyAsync(d.cts, hd.cts.data(), n * 4, hipMemcpyHostToDevice, s));
-    HIP_OK(hipMemcpyAsync(d.lid, hd.lid.data(), n, hipMemcpyHostToDevice, s));
-    HIP_OK(hipMemsetAsync(d.dl, 0, n * 4, s));
-    HIP_OK(hipMemsetAsync(d.thr, 0, n * 4, s));
-    HIP_OK(hipMemsetAsync(d.cx, 0, n * 8, s));
-    HIP_OK(hipMemsetAsync(d.cy, 0, n * 8, s));
-    HIP_OK(hipMemsetAsync(d.cz, 0, n * 8, s));
-    HIP_OK(hipMemsetAsync(d.win, 0xFF, n * 4, s));
-    HIP_OK(hipMemsetAsync(d.lm, 0xFF, n * 8, s));
     if (!hd.nloc.empty()) {
         HIP_OK(hipMemcpyAsync(nloc, hd.nloc.data(), hd.nloc.size() * 4, hipMemcpyHostToDevice, s));
         HIP_OK(hipMemcpyAsync(ntime, hd.ntime.data(), hd.ntime.size() * 4, hipMemcpyHostToDevice, s));
@@ -177,13 +203,7 @@ void build_device_state(exaca_state *h) {
     HIP_OK(hipMemcpyAsync(seed_d, seed.data(), seed.size() * 4, hipMemcpyHostToDevice, s));
     HIP_OK(hipMemcpyAsync(reach_d, reach.data(), reach.size() * 8, hipMemcpyHostToDevice, s));
     HIP_OK(hipMemcpyAsync(grow_d, grow.data(), grow.size() * 8, hipMemcpyHostToDevice, s));
-    const std::vector<uint32_t> bits = active_bits(hd);
-    HIP_OK(hipMemcpyAsync(d.bits, bits.data(), bits.size() * 4, hipMemcpyHostToDevice, s));
-    if (!hd.active.empty()) {
-        HIP_OK(hipMemcpyAsync(d.list[0], hd.active.data(), hd.active.size() * 4, hipMemcpyHostToDevice, s));
-        launch_seed_init(d, d.list[0], static_cast<uint32_t>(hd.active.size()), s);
-        HIP_OK(hipGetLastError());
-    }
+    upload_cells(d, hd, s);
     StepCtl c;
     std::memset(&c, 0, sizeof c);
     c.cycle[0] = 1;
@@ -447,66 +467,11 @@ int exaca_run_layer(exaca_state *h, int layer, int max_cycles, int verbose, int 
 int exaca_download(exaca_state *h, int field, void *dst, size_t dst_bytes) {
     return guarded([&] {
         if (!h || !dst) throw std::invalid_argument("null argument");
-        const size_t n = static_cast<size_t>(h->dv.ncell);
-        const exaca::DomainView &d = h->dv;
-        auto need = [&](size_t b) {
-            if (dst_bytes < b) throw std::invalid_argument("destination too small");
-        };
-        HIP_OK(hipStreamSynchronize(h->stream));
-        switch (field) {
-        case EXACA_FIELD_CELLTYPE:
-        case EXACA_FIELD_CELLTYPE_RAW: {
-            std::vector<uint8_t> t(n);
-            HIP_OK(hipMemcpy(t.data(), d.ct, n, hipMemcpyDeviceToHost));
-            if (field == EXACA_FIELD_CELLTYPE_RAW) {
-                need(n);
-                std::memcpy(dst, t.data(), n);
-            }
-            else {
-                need(4 * n);
-                int32_t *o = static_cast<int32_t *>(dst);
-                for (size_t q = 0; q < n; q++) o[q] = t[q] & exaca::kTypeMask;
-            }
-            break;
-        }
-        case EXACA_FIELD_GRAINID:
-            need(4 * n);
-            HIP_OK(hipMemcpy(dst, d.gid, 4 * n, hipMemcpyDeviceToHost));
-            break;
-        case EXACA_FIELD_CRITTIMESTEP:
-            need(4 * n);
-            HIP_OK(hipMemcpy(dst, d.cts, 4 * n, hipMemcpyDeviceToHost));
-            break;
-        case EXACA_FIELD_DIAGONAL:
-            need(4 * n);
-            HIP_OK(hipMemcpy(dst, d.dl, 4 * n, hipMemcpyDeviceToHost));
-            break;
-        case EXACA_FIELD_LAYERID: {
-            need(4 * n);
-            std::vector<int8_t> t(n);
-            HIP_OK(hipMemcpy(t.data(), d.lid, n, hipMemcpyDeviceToHost));
-            int32_t *o = static_cast<int32_t *>(dst);
-            for (size_t q = 0; q < n; q++) o[q] = t[q];
-            break;
-        }
-        case EXACA_FIELD_DOCENTER: {
-            need(12 * n);
-            std::vector<double> t(n);
-            float *o = static_cast<float *>(dst);
-            double *src[3] = {d.cx, d.cy, d.cz};
-            for (int a = 0; a < 3; a++) {
-                HIP_OK(hipMemcpy(t.data(), src[a], 8 * n, hipMemcpyDeviceToHost));
-                for (size_t q = 0; q < n; q++) o[3 * q + a] = static_cast<float>(t[q]);
-            }
-            break;
-        }
-        default:
-            throw std::invalid_argument("unknown field " + std::to_string(field));
-        }
+        exaca::download_field(h->dv, field, dst, dst_bytes, h->stream);
     });
 }
 
-int exaca_query(const exaca_state *h, int64_t out[16]) {
+int exaca_query(const exaca_state *h, int64_t out[18]) {
     return guarded([&] {
         if (!h || !out) throw std::invalid_argument("null argument");
         exaca::StepStats st;
@@ -531,6 +496,8 @@ int exaca_query(const exaca_state *h, int64_t out[16]) {
         out[13] = c.count[h->parity];
         out[14] = h->dv.list_cap;
         out[15] = static_cast<int64_t>(h->bytes);
+        out[16] = static_cast<int64_t>(st.evaluated);
+        out[17] = static_cast<int64_t>(st.masks);
     });
 }
 
@@ -668,7 +635,7 @@ int exaca_run_program(int id, int np, const char *input_file) {
         if (d.print_misorientation || d.print_full) {
             std::cout << "Collecting data on rank 0 and printing to files" << std::endl;
             std::vector<int32_t> gid(static_cast<size_t>(h->dv.ncell));
-            HIP_OK(hipMemcpy(gid.data(), h->dv.gid, gid.size() * 4, hipMemcpyDeviceToHost));
+            exaca::download_field(h->dv, exaca::kFieldGrainId, gid.data(), gid.size() * 4, h->stream);
             exaca::write_vtk_files(d, h->host, gid);
         }
         else {

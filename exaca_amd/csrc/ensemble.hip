@@ -24,7 +24,8 @@
 #include "runtime_util.hpp"
 
 namespace exaca {
-std::vector<uint32_t> active_bits(const HostDomain &hd);
+void upload_cells(const DomainView &d, const HostDomain &hd, hipStream_t s);
+void download_field(const DomainView &d, int field, void *dst, size_t dst_bytes, hipStream_t s);
 }  // namespace exaca
 
 namespace {
@@ -81,9 +82,8 @@ struct Arena {
 size_t domain_bytes(const exaca::HostDomain &h, size_t ngrow) {
     const size_t n = static_cast<size_t>(h.ncell);
     size_t b = 0;
-    b += up(n) + up(4 * n) + up(4 * n) + up(n) + 2 * up(4 * n) + 3 * up(8 * n);  // cell state
-    b += up(8 * n) + up(4 * n);                                                  // liquid masks, claims
-    b += 4 * up(4 * n) + up(4 * kMaxSlices);                                     // frontier, claim lists
+    b += up(sizeof(exaca::Cell) * n) + up(sizeof(exaca::Centre) * n);  // cell records
+    b += 4 * up(4 * n) + up(4 * kMaxSlices);                            // frontier, claim lists
     b += 2 * up(4 * std::max<size_t>(h.nloc.size(), 1)) + up(8 * ngrow);
     b += up(sizeof(exaca::StepCtl)) + up(sizeof(exaca::StepStats));
     const size_t nwords = (n + 31) / 32;
@@ -201,17 +201,8 @@ void admit(exaca_ensemble *e, const exaca_prepared *pr, int *slot_out) {
     d.xoff = hd.xoff;
     d.yoff = hd.yoff;
     d.ncell = hd.ncell;
-    d.ct = reinterpret_cast<uint8_t *>(carve(n));
-    d.gid = reinterpret_cast<int32_t *>(carve(4 * n));
-    d.cts = reinterpret_cast<int32_t *>(carve(4 * n));
-    d.lid = reinterpret_cast<int8_t *>(carve(n));
-    d.dl = reinterpret_cast<float *>(carve(4 * n));
-    d.thr = reinterpret_cast<float *>(carve(4 * n));
-    d.cx = reinterpret_cast<double *>(carve(8 * n));
-    d.cy = reinterpret_cast<double *>(carve(8 * n));
-    d.cz = reinterpret_cast<double *>(carve(8 * n));
-    d.lm = reinterpret_cast<unsigned long long *>(carve(8 * n));
-    d.win = reinterpret_cast<uint32_t *>(carve(4 * n));
+    d.cells = reinterpret_cast<Cell *>(carve(sizeof(Cell) * n));
+    d.centre = reinterpret_cast<Centre *>(carve(sizeof(Centre) * n));
     d.list[0] = reinterpret_cast<uint32_t *>(carve(4 * n));
     d.list[1] = reinterpret_cast<uint32_t *>(carve(4 * n));
     d.claim_cell = reinterpret_cast<uint32_t *>(carve(4 * n));
@@ -240,28 +231,12 @@ void admit(exaca_ensemble *e, const exaca_prepared *pr, int *slot_out) {
     d.status = e->status_d + slot;
 
     hipStream_t s = e->stream;
-    std::vector<uint8_t> ct = hd.ct;
-    for (uint32_t q : hd.active) ct[q] |= kSeedBit;
-    HIP_OK(hipMemcpyAsync(d.ct, ct.data(), n, hipMemcpyHostToDevice, s));
-    HIP_OK(hipMemcpyAsync(d.gid, hd.gid.data(), 4 * n, hipMemcpyHostToDevice, s));
-    HIP_OK(hipMemcpyAsync(d.cts, hd.cts.data(), 4 * n, hipMemcpyHostToDevice, s));
-    HIP_OK(hipMemcpyAsync(d.lid, hd.lid.data(), n, hipMemcpyHostToDevice, s));
-    HIP_OK(hipMemsetAsync(d.dl, 0, 4 * n, s));
-    HIP_OK(hipMemsetAsync(d.thr, 0, 4 * n, s));
-    HIP_OK(hipMemsetAsync(d.win, 0xFF, 4 * n, s));
-    HIP_OK(hipMemsetAsync(d.lm, 0xFF, 8 * n, s));
     if (!hd.nloc.empty()) {
         HIP_OK(hipMemcpyAsync(nloc, hd.nloc.data(), 4 * hd.nloc.size(), hipMemcpyHostToDevice, s));
         HIP_OK(hipMemcpyAsync(ntime, hd.ntime.data(), 4 * hd.ntime.size(), hipMemcpyHostToDevice, s));
     }
     HIP_OK(hipMemcpyAsync(grow, pr->grow.data(), 8 * pr->grow.size(), hipMemcpyHostToDevice, s));
-    const std::vector<uint32_t> bits = exaca::active_bits(hd);
-    HIP_OK(hipMemcpyAsync(d.bits, bits.data(), 4 * bits.size(), hipMemcpyHostToDevice, s));
-    if (!hd.active.empty()) {
-        HIP_OK(hipMemcpyAsync(d.list[0], hd.active.data(), 4 * hd.active.size(), hipMemcpyHostToDevice, s));
-        launch_seed_init(d, d.list[0], static_cast<uint32_t>(hd.active.size()), s);
-        HIP_OK(hipGetLastError());
-    }
+    exaca::upload_cells(d, hd, s);
     // Chunks end on parity 0, so a new domain always starts in slot 0 of its control block.
     StepCtl c;
     std::memset(&c, 0, sizeof c);
@@ -474,23 +449,7 @@ int exaca_ensemble_query(exaca_ensemble *e, int slot, int64_t out[8]) {
 int exaca_ensemble_download(exaca_ensemble *e, int slot, int field, void *dst, size_t dst_bytes) {
     return guarded([&] {
         if (!e || !dst || slot < 0 || slot >= e->slots || !e->used[slot]) throw std::invalid_argument("bad slot");
-        const exaca::DomainView &d = e->views[slot];
-        const size_t n = static_cast<size_t>(d.ncell);
-        HIP_OK(hipStreamSynchronize(e->stream));
-        if (field == EXACA_FIELD_GRAINID) {
-            if (dst_bytes < 4 * n) throw std::invalid_argument("destination too small");
-            HIP_OK(hipMemcpy(dst, d.gid, 4 * n, hipMemcpyDeviceToHost));
-        }
-        else if (field == EXACA_FIELD_CELLTYPE) {
-            if (dst_bytes < 4 * n) throw std::invalid_argument("destination too small");
-            std::vector<uint8_t> t(n);
-            HIP_OK(hipMemcpy(t.data(), d.ct, n, hipMemcpyDeviceToHost));
-            int32_t *o = static_cast<int32_t *>(dst);
-            for (size_t q = 0; q < n; q++) o[q] = t[q] & exaca::kTypeMask;
-        }
-        else {
-            throw std::invalid_argument("ensemble downloads: CELLTYPE or GRAINID");
-        }
+        exaca::download_field(e->views[slot], field, dst, dst_bytes, e->stream);
     });
 }
 

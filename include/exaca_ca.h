@@ -76,10 +76,11 @@ int exaca_run_layer(exaca_state *h, int layer, int max_cycles, int verbose, int 
  * dst_bytes bytes. Replaces the deep_copy at src/runCA.cpp:423-424. */
 int exaca_download(exaca_state *h, int field, void *dst, size_t dst_bytes);
 
-/* Sizes and counters. out[0..15]: X, Y, nz, nzActive, nx, ny, nuclei, last cycle,
+/* Sizes and counters. out[0..17]: X, Y, nz, nzActive, nx, ny, nuclei, last cycle,
  * nucleation events (nn), sum of frontier sizes, captures, solidifications, waiting-entry steps,
- * frontier list length, frontier list capacity, device bytes allocated. */
-int exaca_query(const exaca_state *h, int64_t out[16]);
+ * frontier list length, frontier list capacity, device bytes allocated, critical-length
+ * evaluations, liquid-neighbour masks built. */
+int exaca_query(const exaca_state *h, int64_t out[18]);
 
 /* Releases the handle and its device memory. */
 int exaca_destroy(exaca_state *h);
@@ -127,7 +128,7 @@ int exaca_ensemble_autotune(exaca_ensemble *e, int chunks_per_candidate, int *ch
 /* out[0..7]: final cycle (0 while running), nucleation events, frontier-size sum, captures,
  * solidifications, device error flag, free arena bytes, steps enqueued so far. */
 int exaca_ensemble_query(exaca_ensemble *e, int slot, int64_t out[8]);
-/* CELLTYPE or GRAINID of one slot (whole domain, int32). */
+/* One per-cell field of one slot (whole domain, the layout of exaca_download). */
 int exaca_ensemble_download(exaca_ensemble *e, int slot, int field, void *dst, size_t dst_bytes);
 int exaca_ensemble_destroy(exaca_ensemble *e);
 
