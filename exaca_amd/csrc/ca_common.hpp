@@ -4,17 +4,21 @@
 // reference linear order p = z*X*Y + x*Y + y with y fastest (src/CAupdate.cpp:220-225). Active
 // cells are sparse in that order (a front of a few percent of each plane over a band of planes),
 // so everything a step reads or atomically updates for a cell sits in one aligned 32-byte sector:
-//   lm   uint64  liquid neighbours of an active cell: bits 0-25 now, bits 32-57 as they were at
-//                this step's claim; bit 31 set: not yet known (a new cell, or never active)
-//   dl   float   octahedron half-diagonal (DiagonalLength)
+//   lm   uint64  liquid neighbours of an active cell. Bits 0-25: now; bit 31 set: not yet known
+//                (a new cell, or never active). Bits 32-57: the neighbours captured in the step
+//                whose serial (mod 64) is in bits 58-63.
+//   dl   float   octahedron half-diagonal (DiagonalLength) after the growth of steps up to `win`
 //   thr  float   lower bound on the critical lengths of the still-liquid neighbours (0: unknown)
 //   cts  int32   CritTimeStep (liquidus crossing step)
 //   gid  int32   GrainID
-//   win  uint32  lowest index of a cell that claimed this liquid cell in the current step
+//   win  uint32  liquid cell: lowest index of a cell that claimed it in the current step
 //                (0xFFFFFFFF: unclaimed). A claimed cell is captured in the same step and never
-//                turns liquid again, so a claim word is read in the step that wrote it only.
+//                turns liquid again, so a claim word is read in the step that wrote it only; from
+//                its first step as an active cell on, the word holds the growth index: dl includes
+//                the growth of every executed step up to that many steps past the liquidus time.
 //   lid  int8    LayerID
 //   ct   uint8   cell type (values of src/CAtypes.hpp:11) | kSeedBit for cell-centred octahedra
+//   flags        kCounted
 // The octahedron centre (double, as computed at capture; DOCenter is its float rounding) is a
 // second 32-byte record per cell, read only when a cell evaluates critical lengths or captures.
 // The reference's 26 critical diagonal lengths per cell (104 B) are not stored: they are
@@ -41,6 +45,10 @@ constexpr uint8_t kTypeMask = 0x0F;
 constexpr int kNumOrientations = 10000;
 constexpr unsigned long long kLmUnknown = 1ull << 31;
 constexpr unsigned long long kLmNow = (1ull << 26) - 1;
+constexpr uint16_t kCounted = 1;
+// Steps ahead a cell can be scheduled; a cell with nothing to do for longer is scheduled at the
+// horizon and checks again.
+constexpr uint32_t kAlarmSlots = 64;
 constexpr int kMaxJumps = 4096;
 
 // The 26 neighbour offsets in the reference order: 0-8 at dy = -1, 9-16 in the y plane, 17-25 at
@@ -80,14 +88,17 @@ constexpr OppositeTable kOpposite = make_opposites();
 // step read slot p while the first kernel prepares slot 1-p for the next step.
 struct StepCtl {
     int32_t cycle[2];     // time step number seen by the kernels of a step of parity p
-    uint32_t count[2];    // frontier list lengths
-    uint32_t rlo[2];      // cell-index range [rlo, rhi] that holds every active cell of the step
-    uint32_t rhi[2];      //   (rlo > rhi: none); the frontier rebuild scans only this range
+    uint32_t serial[2];   // steps executed before the step of parity p (its calendar slot)
+    uint32_t count[2];    // lengths of the steps' lists of cells to process
+    uint32_t elo, ehi;    // cell-index range [elo, ehi] holding every active cell (elo > ehi: none)
     uint32_t ncand;       // cells whose last liquid neighbour was captured in the current step
     int32_t njumps;       // cycle jumps taken so far (spot-melt problems)
-    int32_t error;        // nonzero: frontier list overflow or invalid grain
+    int32_t error;        // nonzero: list overflow or invalid grain
     int32_t done;         // set by the device when a domain with autostop finishes
     int32_t final_cycle;  // the check cycle at which it finished
+    int32_t last_cycle;   // the last executed step
+    unsigned long long nactive;     // active cells at the end of the last step
+    unsigned long long nonwaiting;  // of those, past their liquidus time (the reference's numSteer)
     int32_t jump_from[kMaxJumps];  // cycle J before the jump; cycles J+1 .. T-1 were skipped
     int32_t jump_to[kMaxJumps];    // T, the first cycle executed after the jump
 };
@@ -108,6 +119,7 @@ struct StepStats {
 struct CheckResult {
     unsigned long long counts[5];  // superheated, undercooled, active, solid, future-layer liquid
     unsigned long long next_cts;   // min CritTimeStep (as unsigned long) of liquid cells in the layer
+    unsigned long long amin, amax; // least and greatest index of an active cell (amin > amax: none)
 };
 
 struct alignas(32) Cell {
@@ -119,7 +131,7 @@ struct alignas(32) Cell {
     uint32_t win;
     int8_t lid;
     uint8_t ct;
-    uint16_t pad;
+    uint16_t flags;  // kCounted: past its liquidus time and counted in StepCtl::nonwaiting
 };
 static_assert(sizeof(Cell) == 32, "one sector per cell");
 struct alignas(32) Centre {
@@ -136,6 +148,23 @@ enum FieldKind : int {
     kFieldCritTimeStep = 4,
     kFieldLayerId = 5,
     kFieldCellTypeRaw = 6,
+};
+
+// Counters of the current step, one slot per workgroup (part[row * nparts + workgroup]): every
+// workgroup overwrites its own slot, and the step's last kernel sums the rows. Same-address atomics
+// from thousands of workgroups would serialise.
+enum PartRow : int {
+    kPartClaimSol,      // solidified in the capture pass
+    kPartClaimEntered,  // first step past the liquidus time
+    kPartClaimEval,     // critical-length evaluations
+    kPartClaimMask,     // liquid masks built
+    kPartNucSol,        // solidified at nucleation (mask emptied by a nucleus)
+    kPartNucEntered,
+    kPartNucAct,        // nucleation events
+    kPartNucNn,         // nucleation events counted as the reference's nn
+    kPartCap,           // captures
+    kPartSettleSol,     // solidified in the settle pass
+    kPartRows
 };
 
 // Everything a kernel needs to address one CA domain.
@@ -156,13 +185,17 @@ struct DomainView {
     const int32_t *ntime;
     const float *guv;       // 10000 x 9 unit-vector components (GrainUnitVector)
     const float *seed_cdl;  // 10000 x 26 critical diagonal lengths of cell-centred octahedra
-    const double *reach;    // per orientation: c with critical length^2 >= c |x - centre|^2
     const double *grow;     // growth increment after k executed steps below the liquidus
     int32_t ngrow;          // table length; the last entry holds for every larger k
+    int32_t ksat;           // from this k on every increment is the last entry (the 0.045 cap or the
+                            // 210 K saturation)
     StepCtl *ctl;
     StepStats *stats;
-    uint32_t *bits;         // one bit per cell: in the frontier (active)
-    uint32_t *chunk_sum;    // frontier cells per 2048-word chunk of bits, then their exclusive prefix
+    unsigned long long *part;  // kPartRows x nparts per-workgroup counters of the current step
+    uint32_t nparts;
+    uint32_t *alarms;       // kAlarmSlots bitmaps of one bit per cell: process the cell at the step
+                            // whose serial is the bitmap's slot (mod kAlarmSlots)
+    uint32_t *chunk_sum;    // set bits per 2048-word chunk of a bitmap, then their exclusive prefix
     uint32_t nchunks;
     int32_t autostop;       // end the domain on the device at the first check with no liquid left
     long long liquid0;      // liquid cells of the layer at the start

@@ -1,14 +1,19 @@
-// ca_kernels.hip -- the CA time step as memory-bound CDNA4 kernels over a compacted frontier.
+// ca_kernels.hip -- the CA time step as memory-bound CDNA4 kernels over the cells due this step.
 //
 // One step (reference: src/runCA.cpp:291-311) is seven launches on one stream:
 //   k_nucleate  activates the pre-sampled nuclei whose time has come (src/CAupdate.cpp:67-200)
-//   k_claim     for every frontier cell past its liquidus time: growth of its octahedron and a
-//               claim on every liquid neighbour it reaches, written as atomicMin of the
-//               capturer's index into win[]; a cell with no liquid neighbour turns solid
+//   k_claim     for every cell due this step: the growth since it was last looked at, a claim on
+//               every liquid neighbour it reaches (atomicMin of its index into the neighbour's
+//               claim word), and the step at which it is due next; a cell past its liquidus time
+//               with no liquid neighbour turns solid
 //   k_apply     the winner of each claimed cell builds the new decentred octahedron and takes the
 //               cell out of its neighbours' liquid masks
 //   k_settle    a cell whose liquid neighbours were all taken by lower-indexed cells turns solid
-//   k_bits_*    the next frontier: the active cells in index order, from a bitmap
+//   k_bits_*    the list of the cells due at the next step, in index order, from its calendar slot
+// A cell is due when something can happen to it: it is new, its liquidus time has passed, or its
+// half-diagonal reaches the least critical length of its liquid neighbours; a neighbour's capture
+// or nucleation that empties its mask is handled where it happens. Between those steps its growth
+// is a fixed recurrence, applied exactly when it is next looked at.
 // Claim-then-apply reproduces the reference's Serial backend exactly: there pass 2 visits active
 // cells in ascending index order and every capture is visible at once, so a liquid cell goes to
 // the lowest-indexed cell that reaches it, and cell j still "sees" a liquid neighbour only if no
@@ -19,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include "ca_kernels.hpp"
+#include "growth.hpp"
 
 namespace exaca {
 namespace {
@@ -284,8 +290,10 @@ struct CellWords {
     unsigned long long lm;
     float dl, thr;
     int32_t cts, gid;
+    uint32_t win;
     int8_t lid;
     uint8_t ct;
+    uint16_t flags;
 };
 __device__ __forceinline__ CellWords load_cell(const Cell *c) {
     const uint4 a = reinterpret_cast<const uint4 *>(c)[0];
@@ -296,51 +304,138 @@ __device__ __forceinline__ CellWords load_cell(const Cell *c) {
     w.thr = __uint_as_float(a.w);
     w.cts = static_cast<int32_t>(b.x);
     w.gid = static_cast<int32_t>(b.y);
+    w.win = b.z;
     w.lid = static_cast<int8_t>(b.w & 0xFFu);
     w.ct = static_cast<uint8_t>((b.w >> 8) & 0xFFu);
+    w.flags = static_cast<uint16_t>(b.w >> 16);
     return w;
 }
 
-__device__ __forceinline__ void set_active_bit(const DomainView &d, uint32_t q) {
-    atomicOr(&d.bits[q >> 5], 1u << (q & 31));
+// The low word of a cell's liquid-neighbour mask: bits 0-25 and the not-yet-known bit.
+__device__ __forceinline__ uint32_t *mask_word(const DomainView &d, uint32_t q) {
+    return reinterpret_cast<uint32_t *>(&d.cells[q].lm);
 }
-__device__ __forceinline__ void clear_active_bit(const DomainView &d, uint32_t q) {
-    atomicAnd(&d.bits[q >> 5], ~(1u << (q & 31)));
+constexpr uint32_t kMaskUnknown = static_cast<uint32_t>(kLmUnknown);
+constexpr uint32_t kMaskNow = static_cast<uint32_t>(kLmNow);
+
+// ---- Growth between events. A cell's half-diagonal grows by grow[k] at the k-th executed step
+// past its liquidus time (src/CAupdate.cpp:257-261). Between two events of a cell the growth is
+// this fixed recurrence, so it is applied when the cell is next looked at, one float rounding per
+// step exactly as the reference stores it. ----
+
+// k of the step at `cycle` (0 while the cell waits for its liquidus time).
+__device__ __forceinline__ int growth_index(const DomainView &d, int cts, int cycle) {
+    const int k = executed_since(cts, cycle, d.ctl, d.ctl->njumps);
+    return k < 0 ? 0 : k;
+}
+// The growth increments: grow[k] below the plateau index ksat, the plateau value from there on.
+// tab is the table or a copy of its first ksat entries in LDS (the capture pass stages one, as its
+// replays walk the table one dependent step at a time).
+struct Growth {
+    const double *tab;
+    int ksat;
+    double gsat;
+    __device__ __forceinline__ double inc(int k) const { return k < ksat ? tab[k] : gsat; }
+};
+__device__ __forceinline__ Growth growth_of(const DomainView &d) { return Growth{d.grow, d.ksat, d.grow[d.ngrow - 1]}; }
+
+// dl after the growth of the steps with index k0+1 .. k1, one float rounding per step exactly as
+// the reference stores it.
+__device__ float replay_growth(const Growth &g, float dl, int k0, int k1) {
+    int k = k0;
+    for (; k < k1 && k + 1 < g.ksat; k++) dl = grow_step(dl, g.inc(k + 1));
+    if (k < k1) dl = advance_const(dl, k1 - k, g.gsat);
+    return dl;
+}
+// Steps ahead (1 .. kAlarmSlots - 1) at which dl, now at growth index k, first reaches thr; the
+// horizon when it does not get there before.
+__device__ uint32_t steps_until(const Growth &g, float dl, int k, float thr) {
+    const int horizon = static_cast<int>(kAlarmSlots) - 1;
+    int n = 0;
+    while (n < horizon && k + n + 1 < g.ksat) {
+        n++;
+        dl = grow_step(dl, g.inc(k + n));
+        if (dl >= thr) return static_cast<uint32_t>(n);
+    }
+    if (n == horizon) return static_cast<uint32_t>(horizon);
+    return static_cast<uint32_t>(n + reach_const(dl, g.gsat, thr, horizon - n));
+}
+
+// ---- The calendar: cell q is processed at the step of serial s when bit q of slot s mod
+// kAlarmSlots is set. ----
+__device__ __forceinline__ size_t bitmap_words(const DomainView &d) { return static_cast<size_t>((d.ncell + 31) / 32); }
+__device__ __forceinline__ void set_alarm(const DomainView &d, uint32_t serial, uint32_t q) {
+    atomicOr(&d.alarms[(serial % kAlarmSlots) * bitmap_words(d) + (q >> 5)], 1u << (q & 31));
+}
+
+struct StepCounts {
+    unsigned long long nsol, entered, neval, nmask;
+};
+// Workgroup b's value of a counter row for this step (thread 0 holds the block sum).
+__device__ __forceinline__ void put_part(const DomainView &d, int row, uint32_t b, unsigned long long v,
+                                         unsigned long long *sh) {
+    const unsigned long long t = block_sum(v, sh);
+    if (threadIdx.x == 0) d.part[static_cast<size_t>(row) * d.nparts + b] = t;
+}
+
+// Cell r of known mask is past its liquidus time and has no liquid neighbour left: at its visit
+// in this step's capture the reference grows it and turns it solid (src/CAupdate.cpp:722-725).
+__device__ __forceinline__ void solidify(const DomainView &d, const Growth &g, uint32_t r, const CellWords &w, int k) {
+    Cell &c = d.cells[r];
+    c.dl = replay_growth(g, w.dl, static_cast<int>(w.win), k);
+    c.win = static_cast<uint32_t>(k);
+    c.ct = kSolid;
 }
 
 // Every body below works on one domain with workgroups b = 0 .. nb-1; the plain launches use the
 // grid x dimension, the ensemble launches add one grid row per domain slot. A finished domain
 // (device-side termination) keeps its state: its bodies return at once.
+
+// Nucleation (src/CAupdate.cpp:67-200). A nucleus leaves the liquid masks of its neighbours; a
+// neighbour whose mask this empties and that is past its liquidus time finds no liquid neighbour in
+// this step's capture and turns solid here.
 __device__ __forceinline__ void nucleate_body(const DomainView &d, int p, int layer, uint32_t b, uint32_t nb) {
+    __shared__ unsigned long long sh[kBlock / 64];
     StepCtl *ctl = d.ctl;
     if (ctl->done) return;
     const int cycle = ctl->cycle[p];
     if (b == 0 && threadIdx.x == 0) {
         ctl->cycle[1 - p] = cycle + 1;
-        ctl->count[1 - p] = 0;
+        ctl->serial[1 - p] = ctl->serial[p] + 1;
         ctl->ncand = 0;
     }
+    const Grid gr = grid_of(d);
     unsigned long long nn = 0, act = 0;
+    StepCounts n{0, 0, 0, 0};
     for (int i = b * kBlock + threadIdx.x; i < d.nnuclei; i += nb * kBlock) {
         const uint32_t q = static_cast<uint32_t>(d.nloc[i]);
         if (!(cycle >= d.ntime[i] && d.cells[q].ct == kLiquid && d.cells[q].lid <= layer)) continue;
-        const Grid gr = grid_of(d);
         const Coord c = decode(q, gr);
         if (c.x > 0 && c.x < d.X - 1 && c.y > 0 && c.y < d.Y - 1) nn++;
-        d.cells[q].dl = 0.01f;
-        d.cells[q].thr = 0.0f;
+        Cell &cq = d.cells[q];
+        cq.dl = 0.01f;
+        cq.thr = 0.0f;
+        cq.lm = kLmUnknown;
+        cq.flags = 0;
+        cq.ct = kActive | kSeedBit;
         d.centre[q] = Centre{(c.x + d.xoff) + 0.5, (c.y + d.yoff) + 0.5, c.z + 0.5, 0.0};
-        d.cells[q].lm = kLmUnknown;
-        d.cells[q].ct = kActive | kSeedBit;
-        set_active_bit(d, q);
-        // q is no longer liquid for any neighbour
         const uint32_t valid = valid_neighbours(c, gr.X, gr.Y, gr.nzA);
 #pragma unroll
-        for (int l = 0; l < 26; l++)
-            if ((valid >> l) & 1u)
-                atomicAnd(reinterpret_cast<uint32_t *>(&d.cells[q + gr.off[l]].lm), ~(1u << kOpposite.v[l]));
-        atomicMin(&ctl->rlo[p], q);
-        atomicMax(&ctl->rhi[p], q);
+        for (int l = 0; l < 26; l++) {
+            if (!((valid >> l) & 1u)) continue;
+            const uint32_t r = q + gr.off[l];
+            const uint32_t bit = 1u << kOpposite.v[l];
+            const uint32_t old = atomicAnd(mask_word(d, r), ~bit);
+            if ((old & (kMaskUnknown | kMaskNow)) != bit) continue;
+            const CellWords w = load_cell(d.cells + r);
+            if ((w.ct & kTypeMask) != kActive || !(cycle > w.cts && w.lid <= layer)) continue;
+            if (!(w.flags & kCounted)) n.entered++;
+            d.cells[r].flags = w.flags | kCounted;
+            solidify(d, growth_of(d), r, w, growth_index(d, w.cts, cycle));
+            n.nsol++;
+        }
+        atomicMin(&ctl->elo, q);
+        atomicMax(&ctl->ehi, q);
         act++;
         const uint32_t slot = atomicAdd(&ctl->count[p], 1u);
         if (slot < d.list_cap)
@@ -348,39 +443,17 @@ __device__ __forceinline__ void nucleate_body(const DomainView &d, int p, int la
         else
             atomicOr(&ctl->error, 1);
     }
-    if (nn) atomicAdd(&d.stats->nucleated, nn);
-    if (act) atomicAdd(&d.stats->activated, act);
+    put_part(d, kPartNucNn, b, nn, sh);
+    put_part(d, kPartNucAct, b, act, sh);
+    put_part(d, kPartNucSol, b, n.nsol, sh);
+    put_part(d, kPartNucEntered, b, n.entered, sh);
 }
 
-// Pass A, one frontier cell per lane: the liquidus-time gate, growth, and the claims.
-// The liquid neighbours come from the cell's own mask (lm), kept current by the cells that leave
-// the liquid state; a cell whose mask is not yet known (new this step) reads the 26 neighbour
-// types once. A cell past its liquidus time with no liquid neighbour turns solid here (no capture
-// of this step can give it one).
-// thr is a lower bound on the critical length of every neighbour that is still liquid: the
-// smallest critical length (or a proven lower bound of it) over the liquid neighbours the cell did
-// not claim when it last evaluated them. Liquid sets only shrink and a cell's octahedron centre is
-// fixed, so while the half-diagonal stays below thr no neighbour can be captured and the step
-// needs neither the octahedron nor any critical length. A new active cell starts at thr = 0.
-// Cells with claims are appended to the claim lists inside the workgroup's own frontier slice (an
-// LDS counter, no global atomics), and the slice's count goes to slice_claims[b] for pass B.
-//
-// The common case (mask known, half-diagonal below thr) reads one record and writes the grown
-// half-diagonal. The rare cases, a mask to build or critical lengths to evaluate (a few percent of
-// the cells but present in nearly every wave), are queued in LDS and run afterwards with every lane
-// busy, instead of holding their whole wave at one or two active lanes.
-
-constexpr uint32_t kQueueCap = 4 * kBlock;
-constexpr uint32_t kItemNewMask = 0xFFFFFFFFu;  // queue tag: mask not yet known (not a float bit pattern used)
-
-struct ClaimCounts {
-    unsigned long long front, nwait, nsol, neval, nmask;
-};
-
 // Critical lengths of the liquid neighbours of frontier cell j (half-diagonal dlj): claims on
-// those reached; thr becomes the least length (or bound) of the others. Returns the claimed mask.
+// those reached. thr becomes the least length of the others and is returned with the claimed
+// mask.
 __device__ uint32_t evaluate_claims(const DomainView &d, const Grid &gr, uint32_t j, float dlj, uint32_t liquid,
-                                    int32_t gid, uint8_t ct) {
+                                    int32_t gid, uint8_t ct, float &thr) {
     int o = orientation_of(gid);
     if (o < 0) {
         atomicOr(&d.ctl->error, 2);
@@ -407,20 +480,8 @@ __device__ uint32_t evaluate_claims(const DomainView &d, const Grid &gr, uint32_
         const Centre e = d.centre[j];
         const Faces f = faces_of(d.guv + 9 * o);
         const double xp = (c.x + d.xoff) + 0.5, yp = (c.y + d.yoff) + 0.5, zp = c.z + 0.5;
-        // The critical length of a neighbour at distance |e| from the centre is at least
-        // sqrt(reach)|e| (reach from the orientation's deviation from orthonormality, with a
-        // margin far above rounding): below that the exact length is not needed, and the bound,
-        // rounded down to float, stands in for it in thr.
-        const double reach = d.reach[o];
-        const double dl2 = static_cast<double>(dlj) * static_cast<double>(dlj);
         for (uint32_t m = liquid; m; m &= m - 1) {
             const int l = __builtin_ctz(m);
-            const double ex = xp + nb_dx(l) - e.x, ey = yp + nb_dy(l) - e.y, ez = zp + nb_dz(l) - e.z;
-            const double r2 = reach * (ex * ex + ey * ey + ez * ez);
-            if (dl2 < r2) {
-                next = fminf(next, __double2float_rd(sqrt(r2)));
-                continue;
-            }
             const float cdl = critical_length(f, xp, yp, zp, e.x, e.y, e.z, l);
             if (dlj >= cdl) {
                 atomicMin(&d.cells[j + gr.offset(l)].win, j);
@@ -432,151 +493,120 @@ __device__ uint32_t evaluate_claims(const DomainView &d, const Grid &gr, uint32_
         }
     }
     d.cells[j].thr = next;
+    thr = next;
     return claimed;
 }
 
-// The half-diagonal after this step's growth (src/CAupdate.cpp:257-261), stored.
-__device__ __forceinline__ float grow_cell(const DomainView &d, uint32_t j, int t, int cycle, int njumps, float dl0) {
-    int k = executed_since(t, cycle, d.ctl, njumps);
-    k = k < 0 ? 0 : (k < d.ngrow - 1 ? k : d.ngrow - 1);
-    const float dlj = static_cast<float>(static_cast<double>(dl0) + d.grow[k]);
-    d.cells[j].dl = dlj;
-    return dlj;
-}
-
-__device__ __forceinline__ void solidify(const DomainView &d, uint32_t j) {
-    d.cells[j].ct = kSolid;
-    clear_active_bit(d, j);
-}
+// Pass A over the cells due this step (the step's calendar slot plus this step's nuclei), one per
+// lane: the liquidus-time gate, the growth since the cell was last looked at, and the claims.
+// The liquid neighbours come from the cell's mask (lm), kept current by the cells that leave the
+// liquid state; a new active cell builds it once from the 26 neighbour types. A cell past its
+// liquidus time with no liquid neighbour turns solid here (no capture of this step can give it
+// one).
+// thr is a lower bound on the critical length of every neighbour that is still liquid: the
+// smallest critical length over the liquid neighbours the cell did not claim when it last
+// evaluated them. Liquid sets only shrink and a cell's octahedron centre is
+// fixed, so no capture is possible before the half-diagonal reaches thr: the cell is scheduled for
+// that step (or for the end of its wait for the liquidus time) and not looked at before, unless a
+// neighbour's capture or nucleation empties its mask. A new active cell starts at thr = 0.
+// Cells with claims are appended to the claim lists inside the workgroup's own slice of the step's
+// list (an LDS counter, no global atomics), and the slice's count goes to slice_claims[b].
+constexpr int kGrowStage = 4096;  // table entries staged in LDS (32 KB)
 
 __device__ __forceinline__ void claim_body(const DomainView &d, int p, int layer, uint32_t b, uint32_t nb) {
     __shared__ unsigned long long sh[kBlock / 64];
-    __shared__ uint32_t nclaim, nq;
-    __shared__ uint32_t q_cell[kQueueCap], q_tag[kQueueCap], q_liq[kQueueCap];
+    __shared__ uint32_t nclaim;
+    __shared__ double gtab[kGrowStage];
     StepCtl *ctl = d.ctl;
     if (ctl->done) return;
-    if (threadIdx.x == 0) {
-        nclaim = 0;
-        nq = 0;
+    if (threadIdx.x == 0) nclaim = 0;
+    Growth g = growth_of(d);
+    if (d.ksat <= kGrowStage) {
+        for (int k = threadIdx.x; k < d.ksat; k += kBlock) gtab[k] = d.grow[k];
+        g.tab = gtab;
     }
     __syncthreads();
     const int cycle = ctl->cycle[p];
-    const int njumps = ctl->njumps;
+    const uint32_t serial = ctl->serial[p];
     uint32_t begin, end;
     frontier_slice(ctl->count[p], d.list_cap, b, nb, begin, end);
     const uint32_t *list = d.list[p];
     const Grid gr = grid_of(d);
-    ClaimCounts n{0, 0, 0, 0, 0};
-
-    // Runs the queued cells, one per lane, and appends their claims.
-    auto drain = [&]() {
-        const uint32_t total = nq;
-        for (uint32_t k0 = 0; k0 < total; k0 += kBlock) {
-            const uint32_t k = k0 + threadIdx.x;
-            uint32_t j = 0, claimed = 0;
-            if (k < total) {
-                j = q_cell[k];
-                const uint32_t tag = q_tag[k];
-                const CellWords cw = load_cell(d.cells + j);
-                if (tag == kItemNewMask) {
+    StepCounts n{0, 0, 0, 0};
+    for (uint32_t i0 = begin; i0 < end; i0 += kBlock) {
+        const uint32_t i = i0 + threadIdx.x;
+        uint32_t j = 0, claimed = 0;
+        if (i < end) {
+            j = list[i];
+            const CellWords w = load_cell(d.cells + j);
+            if ((w.ct & kTypeMask) == kActive) {
+                const bool waiting = !(cycle > w.cts && w.lid <= layer);
+                const int k = growth_index(d, w.cts, cycle);
+                uint32_t liquid;
+                int kdl;
+                if (w.lm & kLmUnknown) {
                     const Coord c = decode(j, gr);
                     const uint32_t valid = valid_neighbours(c, gr.X, gr.Y, gr.nzA);
-                    unsigned long long liq = 0;
+                    liquid = 0;
 #pragma unroll
                     for (int l = 0; l < 26; l++) {
                         const uint32_t q = ((valid >> l) & 1u) ? j + gr.off[l] : j;
-                        liq |= static_cast<unsigned long long>(d.cells[q].ct == kLiquid) << l;
+                        liquid |= static_cast<uint32_t>(d.cells[q].ct == kLiquid) << l;
                     }
-                    d.cells[j].lm = liq | (liq << 32);
+                    d.cells[j].lm = liquid;
                     n.nmask++;
-                    const uint32_t liquid = static_cast<uint32_t>(liq);
-                    if (!(cycle > cw.cts && cw.lid <= layer)) {
-                        n.nwait++;
-                    }
-                    else {
-                        n.front++;
-                        const float dlj = grow_cell(d, j, cw.cts, cycle, njumps, cw.dl);
-                        if (!liquid) {
-                            solidify(d, j);
-                            n.nsol++;
-                        }
-                        else if (dlj >= cw.thr) {
-                            claimed = evaluate_claims(d, gr, j, dlj, liquid, cw.gid, cw.ct);
-                            n.neval++;
-                        }
-                    }
+                    kdl = waiting ? 0 : k - 1;
                 }
                 else {
-                    claimed = evaluate_claims(d, gr, j, __uint_as_float(tag), q_liq[k], cw.gid, cw.ct);
-                    n.neval++;
+                    liquid = static_cast<uint32_t>(w.lm & kLmNow);
+                    kdl = static_cast<int>(w.win);
                 }
-            }
-            const uint32_t slot = begin + wave_append(claimed != 0, &nclaim);
-            if (claimed) {
-                d.claim_cell[slot] = j;
-                d.claim_bits[slot] = claimed;
-            }
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) nq = 0;
-        __syncthreads();
-    };
-
-    for (uint32_t i0 = begin; i0 < end; i0 += kBlock) {
-        const uint32_t i = i0 + threadIdx.x;
-        bool defer = false;
-        uint32_t j = 0, tag = 0, liquid = 0;
-        if (i < end) {
-            j = list[i];
-            const CellWords cw = load_cell(d.cells + j);
-            if (cw.lm & kLmUnknown) {
-                defer = true;
-                tag = kItemNewMask;
-            }
-            else {
-                liquid = static_cast<uint32_t>(cw.lm & kLmNow);
-                if (liquid != static_cast<uint32_t>(cw.lm >> 32))
-                    d.cells[j].lm = static_cast<unsigned long long>(liquid) | (static_cast<unsigned long long>(liquid) << 32);
-                if (!(cycle > cw.cts && cw.lid <= layer)) {
-                    n.nwait++;
+                Cell &cj = d.cells[j];
+                if (waiting) {
+                    cj.win = static_cast<uint32_t>(kdl);
+                    const int ahead = w.cts + 1 - cycle;
+                    const uint32_t wake = ahead < 1 ? 1u
+                                                    : (ahead < static_cast<int>(kAlarmSlots) - 1 ? static_cast<uint32_t>(ahead)
+                                                                                               : kAlarmSlots - 1);
+                    set_alarm(d, serial + wake, j);
                 }
                 else {
-                    n.front++;
-                    const float dlj = grow_cell(d, j, cw.cts, cycle, njumps, cw.dl);
+                    if (!(w.flags & kCounted)) {
+                        n.entered++;
+                        cj.flags = w.flags | kCounted;
+                    }
+                    const float dlj = replay_growth(g, w.dl, kdl, k);
+                    cj.dl = dlj;
+                    cj.win = static_cast<uint32_t>(k);
                     if (!liquid) {
-                        solidify(d, j);
+                        cj.ct = kSolid;
                         n.nsol++;
                     }
-                    else if (dlj >= cw.thr) {
-                        defer = true;
-                        tag = __float_as_uint(dlj);
+                    else {
+                        float thr = w.thr;
+                        if (dlj >= thr) {
+                            claimed = evaluate_claims(d, gr, j, dlj, liquid, w.gid, w.ct, thr);
+                            n.neval++;
+                        }
+                        // No neighbour left to reach (all claimed now): the captures of this step
+                        // empty the mask and the settle pass decides the cell's fate.
+                        if (thr < __builtin_inff()) set_alarm(d, serial + steps_until(g, dlj, k, thr), j);
                     }
                 }
             }
         }
-        const uint32_t qs = wave_append(defer, &nq);
-        if (defer) {
-            q_cell[qs] = j;
-            q_tag[qs] = tag;
-            q_liq[qs] = liquid;
+        const uint32_t slot = begin + wave_append(claimed != 0, &nclaim);
+        if (claimed) {
+            d.claim_cell[slot] = j;
+            d.claim_bits[slot] = claimed;
         }
-        __syncthreads();
-        if (nq > kQueueCap - kBlock) drain();
     }
     __syncthreads();
-    if (nq) drain();
-    __syncthreads();
     if (threadIdx.x == 0) d.slice_claims[b] = nclaim;
-    unsigned long long t = block_sum(n.front, sh);
-    if (threadIdx.x == 0 && t) atomicAdd(&d.stats->frontier, t);
-    t = block_sum(n.nwait, sh);
-    if (threadIdx.x == 0 && t) atomicAdd(&d.stats->waiting, t);
-    t = block_sum(n.nsol, sh);
-    if (threadIdx.x == 0 && t) atomicAdd(&d.stats->solidified, t);
-    t = block_sum(n.neval, sh);
-    if (threadIdx.x == 0 && t) atomicAdd(&d.stats->evaluated, t);
-    t = block_sum(n.nmask, sh);
-    if (threadIdx.x == 0 && t) atomicAdd(&d.stats->masks, t);
+    put_part(d, kPartClaimSol, b, n.nsol, sh);
+    put_part(d, kPartClaimEntered, b, n.entered, sh);
+    put_part(d, kPartClaimEval, b, n.neval, sh);
+    put_part(d, kPartClaimMask, b, n.nmask, sh);
 }
 
 // Pass B over the cells with claims. Workgroup a takes the claims of the kApplyGroup slices
@@ -584,30 +614,28 @@ __device__ __forceinline__ void claim_body(const DomainView &d, int p, int layer
 // waves and keeps the whole grid resident). A claim whose word holds the claimant's own index is won
 // (the lowest index that reached the cell). The won captures of a round of claimants are numbered
 // by a block scan and then built one per lane: the winner's octahedron gives the captured cell's
-// (src/CAupdate.cpp:353-563, 717), and the captured cell leaves the liquid masks of its
-// neighbours. Only masks that still hold it are updated: the masks are read first, and a cell's
-// bit is cleared by its capture alone, so the read decides exactly. A neighbour whose mask this
-// empties is a solidification candidate of this step, appended (per wave) to the candidate list
-// held in the other frontier buffer.
+// (src/CAupdate.cpp:353-563, 717), the new cell is due at the next step, and the captured cell
+// leaves the liquid masks of its neighbours. Only masks that still hold it are updated: the masks
+// are read first, and a cell's bit is cleared by its capture alone, so the read decides exactly.
+// The update also records the bit among the neighbours captured in this step (bits 32-57, tagged
+// with the step's serial). A neighbour whose mask this empties is a solidification candidate of
+// this step, appended (per wave) to the candidate list held in the other list buffer.
 constexpr uint32_t kApplyGroup = 4;
+constexpr uint32_t kCandStage = 2048;  // candidates staged per workgroup
 __host__ __device__ constexpr uint32_t apply_blocks(uint32_t claim_blocks) {
     return (claim_blocks + kApplyGroup - 1) / kApplyGroup;
 }
-
-// The low word of a cell's liquid-neighbour mask: bits 0-25 and the not-yet-known bit.
-__device__ __forceinline__ uint32_t *mask_word(const DomainView &d, uint32_t q) {
-    return reinterpret_cast<uint32_t *>(&d.cells[q].lm);
-}
-constexpr uint32_t kMaskUnknown = static_cast<uint32_t>(kLmUnknown);
-constexpr uint32_t kMaskNow = static_cast<uint32_t>(kLmNow);
 
 __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t a, uint32_t nbc) {
     __shared__ unsigned long long sh[kBlock / 64];
     __shared__ uint32_t wave_tot[kBlock / 64];
     __shared__ uint32_t pre[kBlock + 1];
     __shared__ uint32_t won_cell[kBlock], won_bits[kBlock];
+    __shared__ uint32_t cstage[kCandStage];
+    __shared__ uint32_t ncstage;
     StepCtl *ctl = d.ctl;
     if (ctl->done) return;
+    if (threadIdx.x == 0) ncstage = 0;
     uint32_t beg[kApplyGroup], upto[kApplyGroup], n = 0;
 #pragma unroll
     for (uint32_t g = 0; g < kApplyGroup; g++) {
@@ -621,10 +649,13 @@ __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t 
         n += cnt;
         upto[g] = n;
     }
+    const uint32_t serial = ctl->serial[p];
+    const unsigned long long tag = static_cast<unsigned long long>(serial % 64u) << 58;
     const Grid gr = grid_of(d);
     uint32_t *cand = d.list[1 - p];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     unsigned long long ncap = 0;
+    uint32_t qlo = 0xFFFFFFFFu, qhi = 0;
     for (uint32_t k0 = 0; k0 < n; k0 += kBlock) {
         // Won claims of this round's claimants, and their block-wide exclusive prefix.
         const uint32_t k = k0 + threadIdx.x;
@@ -686,78 +717,137 @@ __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t 
                 const double zp = (c.z + nb_dz(l)) + 0.5;
                 const Octahedron oc = new_octahedron(d.guv + 9 * o, static_cast<float>(e.x), static_cast<float>(e.y),
                                                      static_cast<float>(e.z), xp, yp, zp);
-                d.cells[q].gid = h;
-                d.cells[q].dl = static_cast<float>(oc.dl);
-                d.cells[q].thr = 0.0f;
+                Cell &cq = d.cells[q];
+                cq.gid = h;
+                cq.dl = static_cast<float>(oc.dl);
+                cq.thr = 0.0f;
+                cq.lm = kLmUnknown;
+                cq.flags = 0;
+                cq.ct = kActive;
                 d.centre[q] = Centre{oc.cx, oc.cy, oc.cz, 0.0};
-                d.cells[q].lm = kLmUnknown;
-                d.cells[q].ct = kActive;
-                set_active_bit(d, q);
+                set_alarm(d, serial + 1, q);
                 ncap++;
-                const Coord cq{c.x + nb_dx(l), c.y + nb_dy(l), c.z + nb_dz(l)};
-                const uint32_t valid = valid_neighbours(cq, gr.X, gr.Y, gr.nzA);
-                uint32_t mw[26];
+                qlo = q < qlo ? q : qlo;
+                qhi = q > qhi ? q : qhi;
+                const Coord cqc{c.x + nb_dx(l), c.y + nb_dy(l), c.z + nb_dz(l)};
+                const uint32_t valid = valid_neighbours(cqc, gr.X, gr.Y, gr.nzA);
+                unsigned long long mw[26];
 #pragma unroll
-                for (int k = 0; k < 26; k++) mw[k] = ((valid >> k) & 1u) ? *mask_word(d, q + gr.off[k]) : kMaskUnknown;
+                for (int kk = 0; kk < 26; kk++) mw[kk] = ((valid >> kk) & 1u) ? d.cells[q + gr.off[kk]].lm : kLmUnknown;
 #pragma unroll
-                for (int k = 0; k < 26; k++) {
-                    const uint32_t bit = 1u << kOpposite.v[k];
-                    if ((mw[k] & (kMaskUnknown | bit)) == bit) {
-                        const uint32_t old = atomicAnd(mask_word(d, q + gr.off[k]), ~bit);
-                        emptied |= static_cast<uint32_t>((old & (kMaskUnknown | kMaskNow)) == bit) << k;
+                for (int kk = 0; kk < 26; kk++) {
+                    const unsigned long long bit = 1ull << kOpposite.v[kk];
+                    unsigned long long old = mw[kk];
+                    if ((old & (kLmUnknown | bit)) != bit) continue;
+                    unsigned long long *word = &d.cells[q + gr.off[kk]].lm;
+                    while (true) {
+                        const unsigned long long seen = (old & (0x3Full << 58)) == tag ? (old >> 32) & kLmNow : 0ull;
+                        const unsigned long long next = ((old & 0xFFFFFFFFull) & ~bit) | tag | ((seen | bit) << 32);
+                        const unsigned long long prev = atomicCAS(word, old, next);
+                        if (prev == old) break;
+                        old = prev;
                     }
+                    emptied |= static_cast<uint32_t>((old & (kLmUnknown | kLmNow)) == bit) << kk;
                 }
             }
-            // Candidates: one scan and one atomic per wave.
+            // Candidates: staged in LDS (one LDS atomic per wave); a wave that finds the stage
+            // full appends to the global list directly.
             const uint32_t ce = __builtin_popcount(emptied);
             uint32_t ctot;
             const uint32_t cincl = wave_inclusive_scan(ce, ctot);
             if (ctot) {
                 uint32_t cbase = 0;
-                if (lane == 63) cbase = atomicAdd(&ctl->ncand, ctot);
+                if (lane == 63) cbase = atomicAdd(&ncstage, ctot);
                 cbase = __shfl(cbase, 63, 64);
+                const bool staged = cbase + ctot <= kCandStage;
+                if (!staged) {
+                    if (lane == 63) cbase = atomicAdd(&ctl->ncand, ctot);
+                    cbase = __shfl(cbase, 63, 64);
+                }
                 uint32_t pos = cbase + cincl - ce;
-                for (uint32_t r = emptied; r; r &= r - 1, pos++)
-                    if (pos < d.list_cap) cand[pos] = q + gr.offset(__builtin_ctz(r));
+                for (uint32_t r = emptied; r; r &= r - 1, pos++) {
+                    const uint32_t cell = q + gr.offset(__builtin_ctz(r));
+                    if (staged)
+                        cstage[pos] = cell;
+                    else if (pos < d.list_cap)
+                        cand[pos] = cell;
+                }
             }
         }
         __syncthreads();
     }
-    const unsigned long long t = block_sum(ncap, sh);
-    if (threadIdx.x == 0 && t) atomicAdd(&d.stats->captures, t);
+    // The staged candidates go to the global list with one atomic per workgroup.
+    __shared__ uint32_t cglobal;
+    const uint32_t staged = ncstage < kCandStage ? ncstage : kCandStage;
+    if (threadIdx.x == 0) cglobal = staged ? atomicAdd(&ctl->ncand, staged) : 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < staged; i += kBlock)
+        if (cglobal + i < d.list_cap) cand[cglobal + i] = cstage[i];
+    put_part(d, kPartCap, a, ncap, sh);
+    // New cells widen the range of active cells (least and greatest index of this workgroup).
+    __shared__ uint32_t lohi[2 * (kBlock / 64)];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t a2 = __shfl_down(qlo, o, 64), b2 = __shfl_down(qhi, o, 64);
+        qlo = a2 < qlo ? a2 : qlo;
+        qhi = b2 > qhi ? b2 : qhi;
+    }
+    if (lane == 0) {
+        lohi[2 * wave] = qlo;
+        lohi[2 * wave + 1] = qhi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kBlock / 64; w++) {
+            qlo = lohi[2 * w] < qlo ? lohi[2 * w] : qlo;
+            qhi = lohi[2 * w + 1] > qhi ? lohi[2 * w + 1] : qhi;
+        }
+        // the range rarely grows: most workgroups need no atomic
+        if (qlo <= qhi) {
+            if (qlo < ctl->elo) atomicMin(&ctl->elo, qlo);
+            if (qhi > ctl->ehi) atomicMax(&ctl->ehi, qhi);
+        }
+    }
 }
 
 // Pass C over the candidates: a cell whose every claim-time liquid neighbour was captured in this
-// step by a lower-indexed cell has no liquid neighbour left when the reference visits it in index
-// order, so it turns solid now (src/CAupdate.cpp:333-334, 722-725); a capture by itself or by a
-// later cell leaves it active. A cell not past its liquidus time is not visited this step.
+// step (bits 32-57 of its mask, tagged with this step) by a lower-indexed cell has no liquid
+// neighbour left when the reference visits it in index order, so it grows and turns solid now
+// (src/CAupdate.cpp:333-334, 722-725); a capture by itself or by a later cell leaves it active,
+// with no liquid neighbour: it turns solid at the next step. A cell not past its liquidus time is
+// not visited this step.
 __device__ __forceinline__ void settle_body(const DomainView &d, int p, int layer, uint32_t b, uint32_t nb) {
     __shared__ unsigned long long sh[kBlock / 64];
     StepCtl *ctl = d.ctl;
     if (ctl->done) return;
     const int cycle = ctl->cycle[p];
+    const uint32_t serial = ctl->serial[p];
+    const unsigned long long tag = static_cast<unsigned long long>(serial % 64u) << 58;
     const uint32_t n = ctl->ncand < d.list_cap ? ctl->ncand : d.list_cap;
     const uint32_t *cand = d.list[1 - p];
     const Grid gr = grid_of(d);
-    unsigned long long nsol = 0;
+    StepCounts c{0, 0, 0, 0};
     for (uint32_t i = b * kBlock + threadIdx.x; i < n; i += nb * kBlock) {
         const uint32_t r = cand[i];
-        if ((d.cells[r].ct & kTypeMask) != kActive) continue;
-        if (!(cycle > d.cells[r].cts && d.cells[r].lid <= layer)) continue;
+        const CellWords w = load_cell(d.cells + r);
+        if ((w.ct & kTypeMask) != kActive) continue;
+        if (!(cycle > w.cts && w.lid <= layer)) continue;
+        if ((w.lm & (0x3Full << 58)) != tag) continue;
         bool solid = true;
-        for (uint32_t m = static_cast<uint32_t>(d.cells[r].lm >> 32); m; m &= m - 1)
+        for (uint32_t m = static_cast<uint32_t>((w.lm >> 32) & kLmNow); m; m &= m - 1)
             if (d.cells[r + gr.offset(__builtin_ctz(m))].win >= r) {
                 solid = false;
                 break;
             }
         if (solid) {
-            d.cells[r].ct = kSolid;
-            clear_active_bit(d, r);
-            nsol++;
+            solidify(d, growth_of(d), r, w, growth_index(d, w.cts, cycle));
+            c.nsol++;
+        }
+        else {
+            set_alarm(d, serial + 1, r);
         }
     }
-    const unsigned long long t = block_sum(nsol, sh);
-    if (threadIdx.x == 0 && t) atomicAdd(&d.stats->solidified, t);
+    put_part(d, kPartSettleSol, b, c.nsol, sh);
 }
 
 // Device-side end of a domain with autostop: at a check step (cycle a multiple of 1000) with no
@@ -774,49 +864,82 @@ __device__ __forceinline__ void maybe_stop(const DomainView &d, StepCtl *ctl, in
     }
 }
 
-// The next frontier is every set bit of `bits` in ascending cell index, so a wave's cells are
-// neighbours in the reference's linear order. Three passes over the bitmap chunks that can hold an
-// active cell: per-chunk counts, one exclusive scan of the chunk counts, and the write.
-// Every active cell of a step lies in [rlo, rhi]: the write sets it to the first and last cell of
-// the new frontier widened by the largest neighbour offset (captures stay within it), and
-// nucleation widens it to each new nucleus.
+// The list of the next step is every set bit of its calendar slot, in ascending cell index, over
+// the chunks of the range of active cells; the slot is cleared as it is read. Three passes: per-chunk
+// counts, one exclusive scan of the chunk counts, and the write.
 constexpr uint32_t kChunkWords = 8 * kBlock;
 constexpr uint32_t kChunkCells = 32 * kChunkWords;
 
-__device__ __forceinline__ bool chunk_range(const DomainView &d, int p, uint32_t &c0, uint32_t &c1) {
-    const uint32_t lo = d.ctl->rlo[p];
-    uint32_t hi = d.ctl->rhi[p];
+__device__ __forceinline__ bool chunk_range(const DomainView &d, uint32_t &c0, uint32_t &c1) {
+    const uint32_t lo = d.ctl->elo;
+    uint32_t hi = d.ctl->ehi;
     if (lo > hi) return false;
     if (hi >= static_cast<uint64_t>(d.ncell)) hi = static_cast<uint32_t>(d.ncell - 1);
     c0 = lo / kChunkCells;
     c1 = hi / kChunkCells;
     return true;
 }
+__device__ __forceinline__ uint32_t *next_slot(const DomainView &d, int p) {
+    return d.alarms + ((d.ctl->serial[p] + 1) % kAlarmSlots) * bitmap_words(d);
+}
 
 __device__ __forceinline__ void bits_count_body(const DomainView &d, int p, uint32_t b, uint32_t nb) {
     __shared__ unsigned long long sh[kBlock / 64];
     if (d.ctl->done) return;
     uint32_t c0, c1;
-    if (!chunk_range(d, p, c0, c1)) return;
-    const uint32_t nwords = static_cast<uint32_t>((d.ncell + 31) / 32);
+    if (!chunk_range(d, c0, c1)) return;
+    const uint32_t nwords = static_cast<uint32_t>(bitmap_words(d));
+    const uint32_t *bits = next_slot(d, p);
     for (uint32_t c = c0 + b; c <= c1; c += nb) {
         uint32_t cnt = 0;
 #pragma unroll
         for (uint32_t k = 0; k < kChunkWords / kBlock; k++) {
             const uint32_t wi = c * kChunkWords + k * kBlock + threadIdx.x;
-            if (wi < nwords) cnt += __builtin_popcount(d.bits[wi]);
+            if (wi < nwords) cnt += __builtin_popcount(bits[wi]);
         }
         const unsigned long long t = block_sum(cnt, sh);
         if (threadIdx.x == 0) d.chunk_sum[c] = static_cast<uint32_t>(t);
     }
 }
 
-__device__ __forceinline__ void bits_scan_body(const DomainView &d, int p) {
+// The step's counters: the per-workgroup slots of each row summed over the grids that wrote them
+// (nb capture and settle workgroups, their apply workgroups, nnb nucleation workgroups).
+struct StepSums {
+    unsigned long long v[kPartRows];
+};
+__device__ __forceinline__ StepSums sum_parts(const DomainView &d, uint32_t nb, uint32_t nnb) {
+    __shared__ unsigned long long red[kPartRows][kBlock / 64];
+    const uint32_t na = apply_blocks(nb);
+    StepSums s;
+#pragma unroll
+    for (int r = 0; r < kPartRows; r++) {
+        const uint32_t n = (r >= kPartNucSol && r <= kPartNucNn) ? nnb : (r == kPartCap ? na : nb);
+        const unsigned long long *row = d.part + static_cast<size_t>(r) * d.nparts;
+        unsigned long long v = 0;
+        for (uint32_t i = threadIdx.x; i < n; i += kBlock) v += row[i];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+        if ((threadIdx.x & 63) == 0) red[r][threadIdx.x >> 6] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kPartRows; r++) {
+        unsigned long long v = 0;
+        for (int w = 0; w < kBlock / 64; w++) v += red[r][w];
+        s.v[r] = v;
+    }
+    return s;
+}
+
+// account: this closes the step (its counters are booked); a rebuild between steps passes 0.
+__device__ __forceinline__ void bits_scan_body(const DomainView &d, int p, int account, uint32_t nb, uint32_t nnb) {
     __shared__ uint32_t part[kBlock];
     StepCtl *ctl = d.ctl;
     if (ctl->done) return;
+    StepSums sums;
+    if (account) sums = sum_parts(d, nb, nnb);
     uint32_t c0 = 1, c1 = 0;
-    const bool any = chunk_range(d, p, c0, c1);
+    const bool any = chunk_range(d, c0, c1);
     const uint32_t nc = any ? c1 - c0 + 1 : 0;
     const uint32_t per = (nc + kBlock - 1) / kBlock;
     const uint32_t s0 = c0 + threadIdx.x * per;
@@ -834,11 +957,27 @@ __device__ __forceinline__ void bits_scan_body(const DomainView &d, int p) {
         }
         ctl->count[1 - p] = static_cast<uint32_t>(run < d.list_cap ? run : d.list_cap);
         if (run > d.list_cap) atomicOr(&ctl->error, 1);
-        if (run == 0) {
-            ctl->rlo[1 - p] = 0xFFFFFFFFu;
-            ctl->rhi[1 - p] = 0;
+        if (account) {
+            const unsigned long long *v = sums.v;
+            const unsigned long long sol = v[kPartClaimSol] + v[kPartNucSol] + v[kPartSettleSol];
+            StepStats *st = d.stats;
+            st->solidified += sol;
+            st->evaluated += v[kPartClaimEval];
+            st->masks += v[kPartClaimMask];
+            st->captures += v[kPartCap];
+            st->nucleated += v[kPartNucNn];
+            st->activated += v[kPartNucAct];
+            // The reference's numSteer of this step: active cells past their liquidus time at the
+            // start of its capture pass.
+            ctl->nonwaiting += v[kPartClaimEntered] + v[kPartNucEntered];
+            const unsigned long long active = ctl->nactive + v[kPartNucAct];
+            st->frontier += ctl->nonwaiting;
+            st->waiting += active - ctl->nonwaiting;
+            ctl->nonwaiting -= sol;
+            ctl->nactive = active + v[kPartCap] - sol;
+            ctl->last_cycle = ctl->cycle[p];
+            maybe_stop(d, ctl, p);
         }
-        maybe_stop(d, ctl, p);
     }
     __syncthreads();
     uint32_t run = part[threadIdx.x];
@@ -858,18 +997,20 @@ __device__ __forceinline__ void bits_write_body(const DomainView &d, int p, uint
     StepCtl *ctl = d.ctl;
     if (ctl->done) return;
     uint32_t c0, c1;
-    if (!chunk_range(d, p, c0, c1)) return;
-    const uint32_t nwords = static_cast<uint32_t>((d.ncell + 31) / 32);
-    const uint32_t total = ctl->count[1 - p];
-    const uint32_t reach = static_cast<uint32_t>(d.X) * static_cast<uint32_t>(d.Y) + static_cast<uint32_t>(d.Y) + 1;
-    const uint32_t last = static_cast<uint32_t>(d.ncell - 1);
+    if (!chunk_range(d, c0, c1)) return;
+    const uint32_t nwords = static_cast<uint32_t>(bitmap_words(d));
+    uint32_t *bits = next_slot(d, p);
     uint32_t *next = d.list[1 - p];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (uint32_t c = c0 + b; c <= c1; c += nb) {
         uint32_t base = d.chunk_sum[c];
         for (uint32_t r = 0; r < kChunkWords / kBlock; r++) {
             const uint32_t wi = c * kChunkWords + r * kBlock + threadIdx.x;
-            const uint32_t w = wi < nwords ? d.bits[wi] : 0u;
+            uint32_t w = 0;
+            if (wi < nwords) {
+                w = bits[wi];
+                if (w) bits[wi] = 0;
+            }
             const uint32_t cnt = __builtin_popcount(w);
             uint32_t wtot;
             const uint32_t incl = wave_inclusive_scan(cnt, wtot);
@@ -885,15 +1026,24 @@ __device__ __forceinline__ void bits_write_body(const DomainView &d, int p, uint
             __syncthreads();
             for (uint32_t k = threadIdx.x; k < rtot; k += kBlock) {
                 const uint32_t pos = base + k;
-                const uint32_t q = stage[k];
-                if (pos < d.list_cap) next[pos] = q;
-                if (pos == 0) ctl->rlo[1 - p] = q > reach ? q - reach : 0;
-                if (pos + 1 == total) ctl->rhi[1 - p] = (last - q > reach) ? q + reach : last;
+                if (pos < d.list_cap) next[pos] = stage[k];
             }
             base += rtot;
             __syncthreads();
         }
     }
+}
+
+// Every active cell due at the next step (after a cycle jump, whose skipped steps the calendar did
+// not count).
+__device__ __forceinline__ void rewake_body(const DomainView &d, int p_next, uint32_t b, uint32_t nb) {
+    StepCtl *ctl = d.ctl;
+    const uint32_t lo = ctl->elo, hi = ctl->ehi;
+    if (lo > hi) return;
+    const uint32_t serial = ctl->serial[p_next];
+    for (unsigned long long q = lo + static_cast<unsigned long long>(b) * kBlock + threadIdx.x; q <= hi && q < static_cast<unsigned long long>(d.ncell);
+         q += static_cast<unsigned long long>(nb) * kBlock)
+        if ((d.cells[q].ct & kTypeMask) == kActive) set_alarm(d, serial, static_cast<uint32_t>(q));
 }
 
 // Plain launches: one domain, passed by value.
@@ -912,9 +1062,14 @@ __global__ __launch_bounds__(kBlock) void k_settle(DomainView d, int p, int laye
 __global__ __launch_bounds__(kBlock) void k_bits_count(DomainView d, int p) {
     bits_count_body(d, p, blockIdx.x, gridDim.x);
 }
-__global__ __launch_bounds__(kBlock) void k_bits_scan(DomainView d, int p) { bits_scan_body(d, p); }
+__global__ __launch_bounds__(kBlock) void k_bits_scan(DomainView d, int p, int account, int nb, int nnb) {
+    bits_scan_body(d, p, account, static_cast<uint32_t>(nb), static_cast<uint32_t>(nnb));
+}
 __global__ __launch_bounds__(kBlock) void k_bits_write(DomainView d, int p) {
     bits_write_body(d, p, blockIdx.x, gridDim.x);
+}
+__global__ __launch_bounds__(kBlock) void k_rewake(DomainView d, int p_next) {
+    rewake_body(d, p_next, blockIdx.x, gridDim.x);
 }
 
 // Ensemble launches: grid row y serves domain slot y of a device-resident table; an empty slot has
@@ -939,9 +1094,9 @@ __global__ __launch_bounds__(kBlock) void k_bits_count_n(const DomainView *v, in
     const DomainView &d = v[blockIdx.y];
     if (d.ctl) bits_count_body(d, p, blockIdx.x, gridDim.x);
 }
-__global__ __launch_bounds__(kBlock) void k_bits_scan_n(const DomainView *v, int p) {
+__global__ __launch_bounds__(kBlock) void k_bits_scan_n(const DomainView *v, int p, int nb, int nnb) {
     const DomainView &d = v[blockIdx.y];
-    if (d.ctl) bits_scan_body(d, p);
+    if (d.ctl) bits_scan_body(d, p, 1, static_cast<uint32_t>(nb), static_cast<uint32_t>(nnb));
 }
 __global__ __launch_bounds__(kBlock) void k_bits_write_n(const DomainView *v, int p) {
     const DomainView &d = v[blockIdx.y];
@@ -952,11 +1107,15 @@ __global__ __launch_bounds__(kBlock) void k_check(DomainView d, int cycle, int l
     __shared__ unsigned long long sh[kBlock / 64];
     __shared__ unsigned long long shmin[kBlock / 64];
     unsigned long long c[5] = {0, 0, 0, 0, 0};
-    unsigned long long next = ~0ull;
+    unsigned long long next = ~0ull, amin = ~0ull, amax = 0;
     const long long active_end = static_cast<long long>(d.X) * d.Y * d.nzA;
     for (long long q = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x; q < d.ncell;
          q += static_cast<long long>(gridDim.x) * kBlock) {
         const int t = d.cells[q].ct & kTypeMask;
+        if (t == kActive) {
+            amin = static_cast<unsigned long long>(q) < amin ? static_cast<unsigned long long>(q) : amin;
+            amax = static_cast<unsigned long long>(q) > amax ? static_cast<unsigned long long>(q) : amax;
+        }
         if (d.cells[q].lid == layer) {
             if (t == kLiquid) {
                 const int crit = d.cells[q].cts;
@@ -992,6 +1151,17 @@ __global__ __launch_bounds__(kBlock) void k_check(DomainView d, int cycle, int l
         for (int q = 1; q < kBlock / 64; q++) m = shmin[q] < m ? shmin[q] : m;
         if (m != ~0ull) atomicMin(&out->next_cts, m);
     }
+    // range of the active cells, one atomic per wave
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long a = __shfl_down(amin, o, 64), b = __shfl_down(amax, o, 64);
+        amin = a < amin ? a : amin;
+        amax = b > amax ? b : amax;
+    }
+    if ((threadIdx.x & 63) == 0 && amin <= amax) {
+        atomicMin(&out->amin, amin);
+        atomicMax(&out->amax, amax);
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_seed_init(DomainView d, const uint32_t *cells, uint32_t n) {
@@ -1010,7 +1180,7 @@ __global__ __launch_bounds__(kBlock) void k_pack(DomainView d, const uint8_t *ct
     for (long long q = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x; q < d.ncell;
          q += static_cast<long long>(gridDim.x) * kBlock) {
         uint4 *c = reinterpret_cast<uint4 *>(d.cells + q);
-        c[0] = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u);
+        c[0] = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u);  // mask not yet known; no octahedron
         c[1] = make_uint4(static_cast<uint32_t>(cts[q]), static_cast<uint32_t>(gid[q]), kClaimNone,
                           static_cast<uint32_t>(static_cast<uint8_t>(lid[q])) | (static_cast<uint32_t>(ct[q]) << 8));
         d.centre[q] = Centre{0.0, 0.0, 0.0, 0.0};
@@ -1026,7 +1196,13 @@ __global__ __launch_bounds__(kBlock) void k_extract(DomainView d, int field, voi
         case kFieldCellType: static_cast<int32_t *>(out)[q] = c.ct & kTypeMask; break;
         case kFieldCellTypeRaw: static_cast<uint8_t *>(out)[q] = c.ct; break;  // uint8
         case kFieldGrainId: static_cast<int32_t *>(out)[q] = c.gid; break;
-        case kFieldDiagonal: static_cast<float *>(out)[q] = c.dl; break;  // float
+        case kFieldDiagonal: {  // float, with the growth of the steps since the cell was last looked at
+            float dl = c.dl;
+            if ((c.ct & kTypeMask) == kActive && !(c.lm & kLmUnknown))
+                dl = replay_growth(growth_of(d), dl, static_cast<int>(c.win), growth_index(d, c.cts, d.ctl->last_cycle));
+            static_cast<float *>(out)[q] = dl;
+            break;
+        }
         case kFieldCritTimeStep: static_cast<int32_t *>(out)[q] = c.cts; break;
         case kFieldLayerId: static_cast<int32_t *>(out)[q] = c.lid; break;
         case kFieldCentre: {  // float x, y, z
@@ -1084,8 +1260,15 @@ void launch_apply(const DomainView &d, int parity, int layer, int blocks, hipStr
     k_apply<<<apply_blocks(blocks), kBlock, 0, s>>>(d, parity, blocks);
     k_settle<<<blocks, kBlock, 0, s>>>(d, parity, layer);
     k_bits_count<<<cb, kBlock, 0, s>>>(d, parity);
-    k_bits_scan<<<1, kBlock, 0, s>>>(d, parity);
+    k_bits_scan<<<1, kBlock, 0, s>>>(d, parity, 1, blocks, grid_for(d.nnuclei, 1024));
     k_bits_write<<<cb, kBlock, 0, s>>>(d, parity);
+}
+void launch_rewake(const DomainView &d, int parity_next, int blocks, hipStream_t s) {
+    const int cb = static_cast<int>(d.nchunks < static_cast<uint32_t>(blocks) ? d.nchunks : blocks);
+    k_rewake<<<blocks, kBlock, 0, s>>>(d, parity_next);
+    k_bits_count<<<cb, kBlock, 0, s>>>(d, 1 - parity_next);
+    k_bits_scan<<<1, kBlock, 0, s>>>(d, 1 - parity_next, 0, blocks, 0);
+    k_bits_write<<<cb, kBlock, 0, s>>>(d, 1 - parity_next);
 }
 void launch_step_timed(const DomainView &d, int parity, int layer, int blocks, hipStream_t s, hipEvent_t *ev) {
     const int cb = static_cast<int>(d.nchunks < static_cast<uint32_t>(blocks) ? d.nchunks : blocks);
@@ -1107,7 +1290,7 @@ void launch_step_timed(const DomainView &d, int parity, int layer, int blocks, h
     k_bits_count<<<cb, kBlock, 0, s>>>(d, parity);
     rec();
     rec();
-    k_bits_scan<<<1, kBlock, 0, s>>>(d, parity);
+    k_bits_scan<<<1, kBlock, 0, s>>>(d, parity, 1, blocks, grid_for(d.nnuclei, 1024));
     rec();
     rec();
     k_bits_write<<<cb, kBlock, 0, s>>>(d, parity);
@@ -1120,7 +1303,7 @@ void launch_step_n(const DomainView *views, int slots, int parity, int layer, in
     k_apply_n<<<dim3(apply_blocks(blocks), slots), kBlock, 0, s>>>(views, parity, blocks);
     k_settle_n<<<dim3(blocks, slots), kBlock, 0, s>>>(views, parity, layer);
     k_bits_count_n<<<dim3(blocks, slots), kBlock, 0, s>>>(views, parity);
-    k_bits_scan_n<<<dim3(1, slots), kBlock, 0, s>>>(views, parity);
+    k_bits_scan_n<<<dim3(1, slots), kBlock, 0, s>>>(views, parity, blocks, nuc_blocks);
     k_bits_write_n<<<dim3(blocks, slots), kBlock, 0, s>>>(views, parity);
 }
 void launch_check(const DomainView &d, int cycle, int layer, CheckResult *out, hipStream_t s) {

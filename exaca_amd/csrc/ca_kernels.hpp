@@ -15,6 +15,9 @@ void launch_claim(const DomainView &d, int parity, int layer, int blocks, hipStr
 // CellCapture pass B: winners build new octahedra, cells left without a liquid neighbour by
 // lower-indexed captures solidify, and the next frontier is written (src/CAupdate.cpp:353-725).
 void launch_apply(const DomainView &d, int parity, int layer, int blocks, hipStream_t s);
+// Every active cell due at the next step, and that step's list rebuilt: after a cycle jump, whose
+// skipped steps the calendar did not count. parity_next is the parity of the next step.
+void launch_rewake(const DomainView &d, int parity_next, int blocks, hipStream_t s);
 // One step of every domain in a device-resident table of `slots` views (grid row = slot).
 void launch_step_n(const DomainView *views, int slots, int parity, int layer, int nuc_blocks, int blocks,
                    hipStream_t s);

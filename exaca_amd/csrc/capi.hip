@@ -27,17 +27,28 @@ thread_local std::string g_last_error;
 }
 
 namespace exaca {
-std::vector<uint32_t> active_bits(const HostDomain &hd) {
-    std::vector<uint32_t> b((static_cast<size_t>(hd.ncell) + 31) / 32, 0u);
-    for (uint32_t q : hd.active) b[q >> 5] |= 1u << (q & 31);
-    return b;
+// The control block of a domain before its first step: the initially active cells are the first
+// step's list, and the range of active cells is theirs.
+StepCtl initial_ctl(const HostDomain &hd) {
+    StepCtl c;
+    std::memset(&c, 0, sizeof c);
+    c.cycle[0] = 1;
+    c.count[0] = static_cast<uint32_t>(hd.active.size());
+    c.elo = 0xFFFFFFFFu;
+    c.ehi = 0;
+    for (uint32_t q : hd.active) {
+        c.elo = std::min(c.elo, q);
+        c.ehi = std::max(c.ehi, q);
+    }
+    c.nactive = hd.active.size();
+    return c;
 }
 
 // Initial cell records of a domain: the compact host arrays are staged in device buffers that are
-// free until the first step (the claim lists and the second frontier list: at least 12 bytes per
-// cell) and packed on the device. The initial frontier goes to list[0], with the cell-centred
-// octahedra of the initially active cells. Synchronises the stream (the host sources are
-// temporaries).
+// free until the first step (the claim lists and the second list: at least 12 bytes per cell) and
+// packed on the device. The initially active cells are the first step's list (list[0]), with their
+// cell-centred octahedra; the calendar starts empty. Synchronises the stream (the host sources
+// are temporaries).
 void upload_cells(const DomainView &d, const HostDomain &hd, hipStream_t s) {
     const size_t n = static_cast<size_t>(hd.ncell);
     std::vector<uint8_t> ct = hd.ct;
@@ -52,8 +63,7 @@ void upload_cells(const DomainView &d, const HostDomain &hd, hipStream_t s) {
     HIP_OK(hipMemcpyAsync(cts_d, hd.cts.data(), 4 * n, hipMemcpyHostToDevice, s));
     launch_pack(d, ct_d, lid_d, gid_d, cts_d, s);
     HIP_OK(hipGetLastError());
-    const std::vector<uint32_t> bits = active_bits(hd);
-    HIP_OK(hipMemcpyAsync(d.bits, bits.data(), 4 * bits.size(), hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemsetAsync(d.alarms, 0, 4 * kAlarmSlots * ((n + 31) / 32), s));
     if (!hd.active.empty()) {
         HIP_OK(hipMemcpyAsync(d.list[0], hd.active.data(), 4 * hd.active.size(), hipMemcpyHostToDevice, s));
         launch_seed_init(d, d.list[0], static_cast<uint32_t>(hd.active.size()), s);
@@ -85,7 +95,6 @@ void download_field(const DomainView &d, int field, void *dst, size_t dst_bytes,
 
 namespace {
 
-using exaca::active_bits;
 using exaca::DeviceError;
 using exaca::StateError;
 using exaca::guarded;
@@ -164,7 +173,7 @@ void build_device_state(exaca_state *h) {
     d.slice_claims = h->alloc<uint32_t>(h->blocks);
     const size_t nwords = (n + 31) / 32;
     d.nchunks = static_cast<uint32_t>((nwords + 2047) / 2048);
-    d.bits = h->alloc<uint32_t>(nwords);
+    d.alarms = h->alloc<uint32_t>(kAlarmSlots * nwords);
     d.chunk_sum = h->alloc<uint32_t>(d.nchunks);
     d.nnuclei = static_cast<int32_t>(hd.nloc.size());
     int32_t *nloc = h->alloc<int32_t>(hd.nloc.size());
@@ -177,20 +186,19 @@ void build_device_state(exaca_state *h) {
     for (int o = 0; o < kNumOrientations; o++) seed_critical_lengths(&hd.guv[9 * o], &seed[26 * o]);
     float *seed_d = h->alloc<float>(seed.size());
     d.seed_cdl = seed_d;
-    std::vector<double> reach(kNumOrientations);
-    for (int o = 0; o < kNumOrientations; o++) reach[o] = reach_bound(&hd.guv[9 * o]);
-    double *reach_d = h->alloc<double>(reach.size());
-    d.reach = reach_d;
     std::vector<double> grow = growth_table(hd.A, hd.B, hd.C, hd.D, hd.dU, 1 << 25);
     if (grow.size() >= (1u << 25)) throw std::runtime_error("undercooling never saturates within 2^25 steps");
     double *grow_d = h->alloc<double>(grow.size());
     d.grow = grow_d;
     d.ngrow = static_cast<int32_t>(grow.size());
+    d.ksat = growth_plateau(grow);
     h->ctl = h->alloc<StepCtl>(1);
     h->stats = h->alloc<StepStats>(1);
     h->check = h->alloc<CheckResult>(1);
     d.ctl = h->ctl;
     d.stats = h->stats;
+    d.nparts = static_cast<uint32_t>(std::max(h->blocks, 1024));
+    d.part = h->alloc<unsigned long long>(static_cast<size_t>(kPartRows) * d.nparts);
 
     // Initial state; per-cell undercooling is not stored (it is a function of the steps executed
     // below the liquidus time).
@@ -201,17 +209,12 @@ void build_device_state(exaca_state *h) {
     }
     HIP_OK(hipMemcpyAsync(guv, hd.guv.data(), hd.guv.size() * 4, hipMemcpyHostToDevice, s));
     HIP_OK(hipMemcpyAsync(seed_d, seed.data(), seed.size() * 4, hipMemcpyHostToDevice, s));
-    HIP_OK(hipMemcpyAsync(reach_d, reach.data(), reach.size() * 8, hipMemcpyHostToDevice, s));
     HIP_OK(hipMemcpyAsync(grow_d, grow.data(), grow.size() * 8, hipMemcpyHostToDevice, s));
     upload_cells(d, hd, s);
-    StepCtl c;
-    std::memset(&c, 0, sizeof c);
-    c.cycle[0] = 1;
-    c.count[0] = static_cast<uint32_t>(hd.active.size());
-    c.rlo[0] = 0;
-    c.rhi[0] = static_cast<uint32_t>(n - 1);
+    StepCtl c = initial_ctl(hd);
     HIP_OK(hipMemcpyAsync(h->ctl, &c, sizeof c, hipMemcpyHostToDevice, s));
     HIP_OK(hipMemsetAsync(h->stats, 0, sizeof(StepStats), s));
+    HIP_OK(hipMemsetAsync(d.part, 0, 8ull * kPartRows * d.nparts, s));
     HIP_OK(hipStreamSynchronize(s));
 }
 
@@ -228,6 +231,10 @@ void set_next_cycle(exaca_state *h, int next) {
     h->njumps++;
     HIP_OK(hipMemcpyAsync(&c->njumps, &h->njumps, 4, hipMemcpyHostToDevice, h->stream));
     HIP_OK(hipMemcpyAsync(&c->cycle[h->parity], &next, 4, hipMemcpyHostToDevice, h->stream));
+    // The calendar counts executed steps and waits for liquidus times in cycles: every active cell
+    // is due at the first step after the jump, and is scheduled afresh from there.
+    exaca::launch_rewake(h->dv, h->parity, h->blocks, h->stream);
+    HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(h->stream));  // host values above are stack temporaries
     h->cycle = next - 1;
 }
@@ -285,6 +292,8 @@ void device_check(exaca_state *h, int *cycle, int layer, uint64_t counts[5], int
     exaca::CheckResult r;
     std::memset(&r, 0, sizeof r);
     r.next_cts = ~0ull;
+    r.amin = ~0ull;
+    r.amax = 0;
     HIP_OK(hipMemcpyAsync(h->check, &r, sizeof r, hipMemcpyHostToDevice, h->stream));
     exaca::launch_check(h->dv, *cycle, layer, h->check, h->stream);
     HIP_OK(hipGetLastError());
@@ -297,6 +306,14 @@ void device_check(exaca_state *h, int *cycle, int layer, uint64_t counts[5], int
     if (err) throw DeviceError("device error flag " + std::to_string(err) + " (1: frontier overflow, 2: grain 0)");
     if (counts)
         for (int k = 0; k < 5; k++) counts[k] = r.counts[k];
+    // The range of active cells only widens between checks; narrow it to the cells active now.
+    uint32_t range[2] = {0xFFFFFFFFu, 0};
+    if (r.amin <= r.amax) {
+        range[0] = static_cast<uint32_t>(r.amin);
+        range[1] = static_cast<uint32_t>(r.amax);
+    }
+    HIP_OK(hipMemcpyAsync(&h->ctl->elo, range, sizeof range, hipMemcpyHostToDevice, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
     if (verbose)
         std::cout << "cycle = " << *cycle << " Superheated liquid cells = " << r.counts[0]
                   << " Undercooled liquid cells = " << r.counts[1] << " Number of nucleation events this layer "

@@ -25,6 +25,7 @@
 
 namespace exaca {
 void upload_cells(const DomainView &d, const HostDomain &hd, hipStream_t s);
+StepCtl initial_ctl(const HostDomain &hd);
 void download_field(const DomainView &d, int field, void *dst, size_t dst_bytes, hipStream_t s);
 }  // namespace exaca
 
@@ -85,9 +86,9 @@ size_t domain_bytes(const exaca::HostDomain &h, size_t ngrow) {
     b += up(sizeof(exaca::Cell) * n) + up(sizeof(exaca::Centre) * n);  // cell records
     b += 4 * up(4 * n) + up(4 * kMaxSlices);                            // frontier, claim lists
     b += 2 * up(4 * std::max<size_t>(h.nloc.size(), 1)) + up(8 * ngrow);
-    b += up(sizeof(exaca::StepCtl)) + up(sizeof(exaca::StepStats));
+    b += up(sizeof(exaca::StepCtl)) + up(sizeof(exaca::StepStats)) + up(8ull * exaca::kPartRows * kMaxSlices);
     const size_t nwords = (n + 31) / 32;
-    b += up(4 * nwords) + up(4 * ((nwords + 2047) / 2048));
+    b += up(4 * exaca::kAlarmSlots * nwords) + up(4 * ((nwords + 2047) / 2048));  // calendar, chunk sums
     return b;
 }
 
@@ -118,7 +119,6 @@ struct exaca_ensemble {
     // tables shared by every slot (one orientation file per ensemble)
     std::string orientation_file;
     float *guv = nullptr, *seed_cdl = nullptr;
-    double *reach = nullptr;
     std::map<int, hipGraphExec_t> graphs;  // per workgroups-per-slot setting
     long long steps = 0;
 
@@ -129,7 +129,6 @@ struct exaca_ensemble {
         if (status_d) (void)hipFree(status_d);
         if (guv) (void)hipFree(guv);
         if (seed_cdl) (void)hipFree(seed_cdl);
-        if (reach) (void)hipFree(reach);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -145,17 +144,11 @@ void ensure_tables(exaca_ensemble *e, const exaca::HostDomain &h, const std::str
     }
     e->orientation_file = orientation_file;
     std::vector<float> seed(26 * kNumOrientations);
-    std::vector<double> reach(kNumOrientations);
-    for (int o = 0; o < kNumOrientations; o++) {
-        seed_critical_lengths(&h.guv[9 * o], &seed[26 * o]);
-        reach[o] = reach_bound(&h.guv[9 * o]);
-    }
+    for (int o = 0; o < kNumOrientations; o++) seed_critical_lengths(&h.guv[9 * o], &seed[26 * o]);
     HIP_OK(hipMalloc(&e->guv, h.guv.size() * 4));
     HIP_OK(hipMalloc(&e->seed_cdl, seed.size() * 4));
-    HIP_OK(hipMalloc(&e->reach, reach.size() * 8));
     HIP_OK(hipMemcpy(e->guv, h.guv.data(), h.guv.size() * 4, hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(e->seed_cdl, seed.data(), seed.size() * 4, hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(e->reach, reach.data(), reach.size() * 8, hipMemcpyHostToDevice));
 }
 
 hipGraphExec_t graph_for(exaca_ensemble *e, int blocks) {
@@ -214,18 +207,20 @@ void admit(exaca_ensemble *e, const exaca_prepared *pr, int *slot_out) {
     double *grow = reinterpret_cast<double *>(carve(8 * pr->grow.size()));
     const size_t nwords = (n + 31) / 32;
     d.nchunks = static_cast<uint32_t>((nwords + 2047) / 2048);
-    d.bits = reinterpret_cast<uint32_t *>(carve(4 * nwords));
+    d.alarms = reinterpret_cast<uint32_t *>(carve(4 * kAlarmSlots * nwords));
     d.chunk_sum = reinterpret_cast<uint32_t *>(carve(4 * d.nchunks));
     d.ctl = reinterpret_cast<StepCtl *>(carve(sizeof(StepCtl)));
     d.stats = reinterpret_cast<StepStats *>(carve(sizeof(StepStats)));
+    d.nparts = kMaxSlices;
+    d.part = reinterpret_cast<unsigned long long *>(carve(8ull * kPartRows * kMaxSlices));
     d.nnuclei = static_cast<int32_t>(hd.nloc.size());
     d.nloc = nloc;
     d.ntime = ntime;
     d.guv = e->guv;
     d.seed_cdl = e->seed_cdl;
-    d.reach = e->reach;
     d.grow = grow;
     d.ngrow = static_cast<int32_t>(pr->grow.size());
+    d.ksat = exaca::growth_plateau(pr->grow);
     d.autostop = 1;
     d.liquid0 = pr->liquid0;
     d.status = e->status_d + slot;
@@ -238,14 +233,10 @@ void admit(exaca_ensemble *e, const exaca_prepared *pr, int *slot_out) {
     HIP_OK(hipMemcpyAsync(grow, pr->grow.data(), 8 * pr->grow.size(), hipMemcpyHostToDevice, s));
     exaca::upload_cells(d, hd, s);
     // Chunks end on parity 0, so a new domain always starts in slot 0 of its control block.
-    StepCtl c;
-    std::memset(&c, 0, sizeof c);
-    c.cycle[0] = 1;
-    c.count[0] = static_cast<uint32_t>(hd.active.size());
-    c.rlo[0] = 0;
-    c.rhi[0] = static_cast<uint32_t>(n - 1);
+    StepCtl c = exaca::initial_ctl(hd);
     HIP_OK(hipMemcpyAsync(d.ctl, &c, sizeof c, hipMemcpyHostToDevice, s));
     HIP_OK(hipMemsetAsync(d.stats, 0, sizeof(StepStats), s));
+    HIP_OK(hipMemsetAsync(d.part, 0, 8ull * kPartRows * kMaxSlices, s));
     const int32_t zero = 0;
     HIP_OK(hipMemcpyAsync(e->status_d + slot, &zero, 4, hipMemcpyHostToDevice, s));
     HIP_OK(hipMemcpyAsync(e->views_d + slot, &d, sizeof d, hipMemcpyHostToDevice, s));

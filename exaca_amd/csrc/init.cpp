@@ -336,19 +336,6 @@ void seed_critical_lengths(const float *g, float out[26]) {
     }
 }
 
-double reach_bound(const float *g) {
-    double e2 = 0;
-    for (int a = 0; a < 3; a++)
-        for (int b = 0; b < 3; b++) {
-            double dot = 0;
-            for (int c = 0; c < 3; c++) dot += static_cast<double>(g[3 * a + c]) * g[3 * b + c];
-            const double dev = dot - (a == b ? 1.0 : 0.0);
-            e2 += dev * dev;
-        }
-    const double lo = 1.0 - std::sqrt(e2);
-    return lo > 0 ? 0.999 * lo : 0.0;
-}
-
 std::vector<double> growth_table(double A, double B, double C, double D, float dU, int max_entries) {
     // U_k = U_{k-1} + dU in float from U_0 = 0 (src/CAupdate.cpp:236-237; TempInit sets U = 0);
     // increment = min(0.045, max(0, A U^3 + B U^2 + C U + D)) with U clamped at 210 (:257-261).
@@ -366,6 +353,12 @@ std::vector<double> growth_table(double A, double B, double C, double D, float d
         u = next;
     }
     return t;
+}
+
+int growth_plateau(const std::vector<double> &grow) {
+    int k = static_cast<int>(grow.size()) - 1;
+    while (k > 0 && grow[k - 1] == grow.back()) k--;
+    return k < 0 ? 0 : k;
 }
 
 HostDomain initialize(const Deck &d, bool verbose) {

@@ -35,15 +35,11 @@ HostDomain initialize(const Deck &deck, bool verbose);
 // (src/CAupdate.cpp:112-194), evaluated with the host libm exactly as the reference does.
 void seed_critical_lengths(const float *guv9, float out26[26]);
 
-// Lower bound c (per orientation, from the unit-vector matrix D) such that every critical length
-// of a captured octahedron satisfies cdl^2 >= c |x - centre|^2: the critical length is
-// sum_i |d_i . e| >= sigma_min(D) |e| and sigma_min^2 >= 1 - ||D^T D - I||_F. A 0.999 factor
-// covers rounding; c = 0 (no bound) when D is far from orthonormal.
-double reach_bound(const float *guv9);
-
 // Growth increment per step after k executed steps below the liquidus, k = 0 .. n-1; the last
 // entry is the saturated value (undercooling clamped at 210 K or no longer increasing).
 std::vector<double> growth_table(double A, double B, double C, double D, float dU, int max_entries);
+// Smallest k from which every entry of the growth table equals the last one.
+int growth_plateau(const std::vector<double> &grow);
 
 // Orientation file reader (src/CAinitialize.cpp:1408-1442).
 std::vector<float> read_orientations(const std::string &path);
