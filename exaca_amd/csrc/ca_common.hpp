@@ -5,8 +5,7 @@
 // cells are sparse in that order (a front of a few percent of each plane over a band of planes),
 // so everything a step reads or atomically updates for a cell sits in one aligned 32-byte sector:
 //   lm   uint64  liquid neighbours of an active cell. Bits 0-25: now; bit 31 set: not yet known
-//                (a new cell, or never active). Bits 32-57: the neighbours captured in the step
-//                whose serial (mod 64) is in bits 58-63.
+//                (a new cell, or never active).
 //   dl   float   octahedron half-diagonal (DiagonalLength) after the growth of steps up to `win`
 //   thr  float   lower bound on the critical lengths of the still-liquid neighbours (0: unknown)
 //   cts  int32   CritTimeStep (liquidus crossing step)

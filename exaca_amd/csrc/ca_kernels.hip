@@ -338,6 +338,18 @@ struct Growth {
     __device__ __forceinline__ double inc(int k) const { return k < ksat ? tab[k] : gsat; }
 };
 __device__ __forceinline__ Growth growth_of(const DomainView &d) { return Growth{d.grow, d.ksat, d.grow[d.ngrow - 1]}; }
+// The same with the table's first ksat entries copied to the workgroup's LDS buffer when they fit
+// (the replays walk the table one dependent step at a time). Every thread of the workgroup calls it.
+constexpr int kGrowStage = 4096;  // table entries staged in LDS (32 KB)
+__device__ __forceinline__ Growth staged_growth(const DomainView &d, double *lds) {
+    Growth g = growth_of(d);
+    if (d.ksat <= kGrowStage) {
+        for (int k = threadIdx.x; k < d.ksat; k += kBlock) lds[k] = d.grow[k];
+        g.tab = lds;
+    }
+    __syncthreads();
+    return g;
+}
 
 // dl after the growth of the steps with index k0+1 .. k1, one float rounding per step exactly as
 // the reference stores it.
@@ -420,13 +432,18 @@ __device__ __forceinline__ void nucleate_body(const DomainView &d, int p, int la
         cq.ct = kActive | kSeedBit;
         d.centre[q] = Centre{(c.x + d.xoff) + 0.5, (c.y + d.yoff) + 0.5, c.z + 0.5, 0.0};
         const uint32_t valid = valid_neighbours(c, gr.X, gr.Y, gr.nzA);
+        // all updates in flight together, then the results
+        uint32_t old[26];
 #pragma unroll
-        for (int l = 0; l < 26; l++) {
-            if (!((valid >> l) & 1u)) continue;
-            const uint32_t r = q + gr.off[l];
-            const uint32_t bit = 1u << kOpposite.v[l];
-            const uint32_t old = atomicAnd(mask_word(d, r), ~bit);
-            if ((old & (kMaskUnknown | kMaskNow)) != bit) continue;
+        for (int l = 0; l < 26; l++)
+            old[l] = ((valid >> l) & 1u) ? atomicAnd(mask_word(d, q + gr.off[l]), ~(1u << kOpposite.v[l])) : 0u;
+        uint32_t emptied = 0;
+#pragma unroll
+        for (int l = 0; l < 26; l++)
+            emptied |= static_cast<uint32_t>(((valid >> l) & 1u) &&
+                                             (old[l] & (kMaskUnknown | kMaskNow)) == (1u << kOpposite.v[l])) << l;
+        for (uint32_t m = emptied; m; m &= m - 1) {
+            const uint32_t r = q + gr.offset(__builtin_ctz(m));
             const CellWords w = load_cell(d.cells + r);
             if ((w.ct & kTypeMask) != kActive || !(cycle > w.cts && w.lid <= layer)) continue;
             if (!(w.flags & kCounted)) n.entered++;
@@ -511,7 +528,6 @@ __device__ uint32_t evaluate_claims(const DomainView &d, const Grid &gr, uint32_
 // neighbour's capture or nucleation empties its mask. A new active cell starts at thr = 0.
 // Cells with claims are appended to the claim lists inside the workgroup's own slice of the step's
 // list (an LDS counter, no global atomics), and the slice's count goes to slice_claims[b].
-constexpr int kGrowStage = 4096;  // table entries staged in LDS (32 KB)
 
 __device__ __forceinline__ void claim_body(const DomainView &d, int p, int layer, uint32_t b, uint32_t nb) {
     __shared__ unsigned long long sh[kBlock / 64];
@@ -520,12 +536,7 @@ __device__ __forceinline__ void claim_body(const DomainView &d, int p, int layer
     StepCtl *ctl = d.ctl;
     if (ctl->done) return;
     if (threadIdx.x == 0) nclaim = 0;
-    Growth g = growth_of(d);
-    if (d.ksat <= kGrowStage) {
-        for (int k = threadIdx.x; k < d.ksat; k += kBlock) gtab[k] = d.grow[k];
-        g.tab = gtab;
-    }
-    __syncthreads();
+    const Growth g = staged_growth(d, gtab);
     const int cycle = ctl->cycle[p];
     const uint32_t serial = ctl->serial[p];
     uint32_t begin, end;
@@ -617,9 +628,8 @@ __device__ __forceinline__ void claim_body(const DomainView &d, int p, int layer
 // (src/CAupdate.cpp:353-563, 717), the new cell is due at the next step, and the captured cell
 // leaves the liquid masks of its neighbours. Only masks that still hold it are updated: the masks
 // are read first, and a cell's bit is cleared by its capture alone, so the read decides exactly.
-// The update also records the bit among the neighbours captured in this step (bits 32-57, tagged
-// with the step's serial). A neighbour whose mask this empties is a solidification candidate of
-// this step, appended (per wave) to the candidate list held in the other list buffer.
+// A neighbour whose mask this empties is a solidification candidate of this step, staged per
+// workgroup and appended to the candidate list held in the other list buffer.
 constexpr uint32_t kApplyGroup = 4;
 constexpr uint32_t kCandStage = 2048;  // candidates staged per workgroup
 __host__ __device__ constexpr uint32_t apply_blocks(uint32_t claim_blocks) {
@@ -631,6 +641,8 @@ __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t 
     __shared__ uint32_t wave_tot[kBlock / 64];
     __shared__ uint32_t pre[kBlock + 1];
     __shared__ uint32_t won_cell[kBlock], won_bits[kBlock];
+    __shared__ int32_t won_gid[kBlock];
+    __shared__ float won_cx[kBlock], won_cy[kBlock], won_cz[kBlock];
     __shared__ uint32_t cstage[kCandStage];
     __shared__ uint32_t ncstage;
     StepCtl *ctl = d.ctl;
@@ -650,7 +662,6 @@ __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t 
         upto[g] = n;
     }
     const uint32_t serial = ctl->serial[p];
-    const unsigned long long tag = static_cast<unsigned long long>(serial % 64u) << 58;
     const Grid gr = grid_of(d);
     uint32_t *cand = d.list[1 - p];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -666,6 +677,13 @@ __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t 
             for (uint32_t g = 1; g < kApplyGroup; g++)
                 if (k >= upto[g - 1]) i = beg[g] + (k - upto[g - 1]);
             j = d.claim_cell[i];
+            // the claimant's grain and centre (float, as the capture uses it) travel through LDS
+            // to the lanes that build its captures
+            const Centre e = d.centre[j];
+            won_gid[threadIdx.x] = d.cells[j].gid;
+            won_cx[threadIdx.x] = static_cast<float>(e.x);
+            won_cy[threadIdx.x] = static_cast<float>(e.y);
+            won_cz[threadIdx.x] = static_cast<float>(e.z);
             for (uint32_t m = d.claim_bits[i]; m; m &= m - 1) {
                 const int l = __builtin_ctz(m);
                 won |= static_cast<uint32_t>(d.cells[j + gr.offset(l)].win == j) << l;
@@ -705,18 +723,23 @@ __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t 
                 const int l = __builtin_ctz(bits);
                 q = jc + gr.offset(l);
                 const Coord c = decode(jc, gr);
-                const int h = d.cells[jc].gid;
+                const Coord cqc{c.x + nb_dx(l), c.y + nb_dy(l), c.z + nb_dz(l)};
+                const uint32_t valid = valid_neighbours(cqc, gr.X, gr.Y, gr.nzA);
+                // the neighbours' masks are loaded before the octahedron is built (they arrive
+                // while it is computed)
+                uint32_t mw[26];
+#pragma unroll
+                for (int kk = 0; kk < 26; kk++) mw[kk] = ((valid >> kk) & 1u) ? *mask_word(d, q + gr.off[kk]) : kMaskUnknown;
+                const int h = won_gid[lo];
                 int o = orientation_of(h);
                 if (o < 0) {
                     atomicOr(&ctl->error, 2);
                     o = 0;
                 }
-                const Centre e = d.centre[jc];
                 const double xp = (c.x + d.xoff + nb_dx(l)) + 0.5;
                 const double yp = (c.y + d.yoff + nb_dy(l)) + 0.5;
                 const double zp = (c.z + nb_dz(l)) + 0.5;
-                const Octahedron oc = new_octahedron(d.guv + 9 * o, static_cast<float>(e.x), static_cast<float>(e.y),
-                                                     static_cast<float>(e.z), xp, yp, zp);
+                const Octahedron oc = new_octahedron(d.guv + 9 * o, won_cx[lo], won_cy[lo], won_cz[lo], xp, yp, zp);
                 Cell &cq = d.cells[q];
                 cq.gid = h;
                 cq.dl = static_cast<float>(oc.dl);
@@ -729,25 +752,20 @@ __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t 
                 ncap++;
                 qlo = q < qlo ? q : qlo;
                 qhi = q > qhi ? q : qhi;
-                const Coord cqc{c.x + nb_dx(l), c.y + nb_dy(l), c.z + nb_dz(l)};
-                const uint32_t valid = valid_neighbours(cqc, gr.X, gr.Y, gr.nzA);
-                unsigned long long mw[26];
-#pragma unroll
-                for (int kk = 0; kk < 26; kk++) mw[kk] = ((valid >> kk) & 1u) ? d.cells[q + gr.off[kk]].lm : kLmUnknown;
+                // updates only where the mask still holds q, all in flight together, then the results
+                uint32_t need = 0;
 #pragma unroll
                 for (int kk = 0; kk < 26; kk++) {
-                    const unsigned long long bit = 1ull << kOpposite.v[kk];
-                    unsigned long long old = mw[kk];
-                    if ((old & (kLmUnknown | bit)) != bit) continue;
-                    unsigned long long *word = &d.cells[q + gr.off[kk]].lm;
-                    while (true) {
-                        const unsigned long long seen = (old & (0x3Full << 58)) == tag ? (old >> 32) & kLmNow : 0ull;
-                        const unsigned long long next = ((old & 0xFFFFFFFFull) & ~bit) | tag | ((seen | bit) << 32);
-                        const unsigned long long prev = atomicCAS(word, old, next);
-                        if (prev == old) break;
-                        old = prev;
-                    }
-                    emptied |= static_cast<uint32_t>((old & (kLmUnknown | kLmNow)) == bit) << kk;
+                    const uint32_t bit = 1u << kOpposite.v[kk];
+                    need |= static_cast<uint32_t>((mw[kk] & (kMaskUnknown | bit)) == bit) << kk;
+                }
+#pragma unroll
+                for (int kk = 0; kk < 26; kk++)
+                    if ((need >> kk) & 1u) mw[kk] = atomicAnd(mask_word(d, q + gr.off[kk]), ~(1u << kOpposite.v[kk]));
+#pragma unroll
+                for (int kk = 0; kk < 26; kk++) {
+                    const uint32_t bit = 1u << kOpposite.v[kk];
+                    emptied |= static_cast<uint32_t>(((need >> kk) & 1u) && (mw[kk] & (kMaskUnknown | kMaskNow)) == bit) << kk;
                 }
             }
             // Candidates: staged in LDS (one LDS atomic per wave); a wave that finds the stage
@@ -810,37 +828,57 @@ __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t 
     }
 }
 
+// The settle pass has a few percent of the capture pass's cells: a quarter of its workgroups.
+__host__ __device__ constexpr uint32_t settle_blocks(uint32_t claim_blocks) {
+    return claim_blocks / 4 > 8 ? claim_blocks / 4 : (claim_blocks < 8 ? claim_blocks : 8);
+}
+
 // Pass C over the candidates: a cell whose every claim-time liquid neighbour was captured in this
-// step (bits 32-57 of its mask, tagged with this step) by a lower-indexed cell has no liquid
-// neighbour left when the reference visits it in index order, so it grows and turns solid now
-// (src/CAupdate.cpp:333-334, 722-725); a capture by itself or by a later cell leaves it active,
-// with no liquid neighbour: it turns solid at the next step. A cell not past its liquidus time is
-// not visited this step.
+// step by a lower-indexed cell has no liquid neighbour left when the reference visits it in index
+// order, so it grows and turns solid now (src/CAupdate.cpp:333-334, 722-725); a capture by itself
+// or by a later cell leaves it active, with no liquid neighbour: it turns solid at the next step. A
+// cell not past its liquidus time is not visited this step.
+// A candidate has no liquid neighbour left, so its claim-time liquid neighbours are exactly its
+// neighbours captured in this step: the active cells whose liquid mask is not yet known (every
+// cell captured before this step built its mask at its first capture pass, which came before this
+// step's captures). Their claim words still name their winners.
 __device__ __forceinline__ void settle_body(const DomainView &d, int p, int layer, uint32_t b, uint32_t nb) {
     __shared__ unsigned long long sh[kBlock / 64];
     StepCtl *ctl = d.ctl;
     if (ctl->done) return;
     const int cycle = ctl->cycle[p];
     const uint32_t serial = ctl->serial[p];
-    const unsigned long long tag = static_cast<unsigned long long>(serial % 64u) << 58;
     const uint32_t n = ctl->ncand < d.list_cap ? ctl->ncand : d.list_cap;
     const uint32_t *cand = d.list[1 - p];
     const Grid gr = grid_of(d);
     StepCounts c{0, 0, 0, 0};
+    __shared__ double gtab[kGrowStage];
+    const Growth g = staged_growth(d, gtab);
     for (uint32_t i = b * kBlock + threadIdx.x; i < n; i += nb * kBlock) {
         const uint32_t r = cand[i];
         const CellWords w = load_cell(d.cells + r);
         if ((w.ct & kTypeMask) != kActive) continue;
         if (!(cycle > w.cts && w.lid <= layer)) continue;
-        if ((w.lm & (0x3Full << 58)) != tag) continue;
-        bool solid = true;
-        for (uint32_t m = static_cast<uint32_t>((w.lm >> 32) & kLmNow); m; m &= m - 1)
-            if (d.cells[r + gr.offset(__builtin_ctz(m))].win >= r) {
-                solid = false;
-                break;
-            }
-        if (solid) {
-            solidify(d, growth_of(d), r, w, growth_index(d, w.cts, cycle));
+        const uint32_t valid = valid_neighbours(decode(r, gr), gr.X, gr.Y, gr.nzA);
+        // the 26 neighbour records loaded together (a neighbour outside the domain loads r itself
+        // and is masked out), then tested with branch-free arithmetic
+        uint32_t lo[26], win[26], tw[26];
+#pragma unroll
+        for (int l = 0; l < 26; l++) {
+            const uint32_t q = ((valid >> l) & 1u) ? r + gr.off[l] : r;
+            lo[l] = *mask_word(d, q);
+            const uint4 hi = reinterpret_cast<const uint4 *>(d.cells + q)[1];  // cts, gid, win, lid|ct|flags
+            win[l] = hi.z;
+            tw[l] = hi.w;
+        }
+        uint32_t keep = 0;
+#pragma unroll
+        for (int l = 0; l < 26; l++) {
+            const uint32_t captured_now = (lo[l] >> 31) & static_cast<uint32_t>(((tw[l] >> 8) & kTypeMask) == kActive);
+            keep |= ((valid >> l) & 1u) & captured_now & static_cast<uint32_t>(win[l] >= r);
+        }
+        if (!keep) {
+            solidify(d, g, r, w, growth_index(d, w.cts, cycle));
             c.nsol++;
         }
         else {
@@ -850,19 +888,6 @@ __device__ __forceinline__ void settle_body(const DomainView &d, int p, int laye
     put_part(d, kPartSettleSol, b, c.nsol, sh);
 }
 
-// Device-side end of a domain with autostop: at a check step (cycle a multiple of 1000) with no
-// liquid left. Liquid cells only ever leave the liquid state by capture or nucleation, so the
-// reference's superheated + undercooled count (src/CAupdate.cpp:738-783) is the initial liquid
-// count minus both.
-__device__ __forceinline__ void maybe_stop(const DomainView &d, StepCtl *ctl, int p) {
-    const int cycle = ctl->cycle[p];
-    if (d.autostop && cycle % 1000 == 0 &&
-        d.liquid0 - static_cast<long long>(d.stats->captures) - static_cast<long long>(d.stats->activated) == 0) {
-        ctl->done = 1;
-        ctl->final_cycle = cycle;
-        if (d.status) *d.status = cycle;
-    }
-}
 
 // The list of the next step is every set bit of its calendar slot, in ascending cell index, over
 // the chunks of the range of active cells; the slot is cleared as it is read. Three passes: per-chunk
@@ -903,7 +928,9 @@ __device__ __forceinline__ void bits_count_body(const DomainView &d, int p, uint
 }
 
 // The step's counters: the per-workgroup slots of each row summed over the grids that wrote them
-// (nb capture and settle workgroups, their apply workgroups, nnb nucleation workgroups).
+// (nb capture workgroups, the apply and settle workgroups that go with them, nnb nucleation
+// workgroups).
+constexpr uint32_t kMaxPartsPerThread = 8;  // nparts <= 8 * kBlock
 struct StepSums {
     unsigned long long v[kPartRows];
 };
@@ -913,10 +940,15 @@ __device__ __forceinline__ StepSums sum_parts(const DomainView &d, uint32_t nb, 
     StepSums s;
 #pragma unroll
     for (int r = 0; r < kPartRows; r++) {
-        const uint32_t n = (r >= kPartNucSol && r <= kPartNucNn) ? nnb : (r == kPartCap ? na : nb);
+        const uint32_t n = (r >= kPartNucSol && r <= kPartNucNn) ? nnb
+                           : (r == kPartCap ? na : (r == kPartSettleSol ? settle_blocks(nb) : nb));
         const unsigned long long *row = d.part + static_cast<size_t>(r) * d.nparts;
         unsigned long long v = 0;
-        for (uint32_t i = threadIdx.x; i < n; i += kBlock) v += row[i];
+#pragma unroll
+        for (uint32_t k = 0; k < kMaxPartsPerThread; k++) {
+            const uint32_t i = threadIdx.x + k * kBlock;
+            if (i < n) v += row[i];
+        }
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
         if ((threadIdx.x & 63) == 0) red[r][threadIdx.x >> 6] = v;
@@ -931,9 +963,13 @@ __device__ __forceinline__ StepSums sum_parts(const DomainView &d, uint32_t nb, 
     return s;
 }
 
-// account: this closes the step (its counters are booked); a rebuild between steps passes 0.
+// account: this closes the step (its counters are booked, and a domain with autostop ends at a
+// check step (cycle a multiple of 1000) with no liquid left: liquid cells only ever leave the
+// liquid state by capture or nucleation, so the reference's superheated + undercooled count
+// (src/CAupdate.cpp:738-783) is the initial liquid count minus both); a rebuild between steps
+// passes 0.
 __device__ __forceinline__ void bits_scan_body(const DomainView &d, int p, int account, uint32_t nb, uint32_t nnb) {
-    __shared__ uint32_t part[kBlock];
+    __shared__ uint32_t part[kBlock / 64];
     StepCtl *ctl = d.ctl;
     if (ctl->done) return;
     StepSums sums;
@@ -945,46 +981,68 @@ __device__ __forceinline__ void bits_scan_body(const DomainView &d, int p, int a
     const uint32_t s0 = c0 + threadIdx.x * per;
     const uint32_t s1 = any ? (s0 + per <= c1 + 1 ? s0 + per : c1 + 1) : 0;
     uint32_t mine = 0;
-    for (uint32_t c = s0; c < s1; c++) mine += d.chunk_sum[c];
-    part[threadIdx.x] = mine;
+    for (uint32_t c0b = s0; c0b < s1; c0b += 8) {
+        uint32_t v8[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) v8[k] = (c0b + k < s1) ? d.chunk_sum[c0b + k] : 0u;
+#pragma unroll
+        for (int k = 0; k < 8; k++) mine += v8[k];
+    }
+    // exclusive scan of the threads' segment sums (wave scans, then the wave totals)
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t wtot;
+    const uint32_t incl = wave_inclusive_scan(mine, wtot);
+    if (lane == 63) part[wave] = wtot;
     __syncthreads();
+    uint32_t base = 0, total = 0;
+    for (int w = 0; w < kBlock / 64; w++) {
+        if (w < wave) base += part[w];
+        total += part[w];
+    }
     if (threadIdx.x == 0) {
-        unsigned long long run = 0;
-        for (int t = 0; t < kBlock; t++) {
-            const uint32_t v = part[t];
-            part[t] = static_cast<uint32_t>(run);
-            run += v;
-        }
-        ctl->count[1 - p] = static_cast<uint32_t>(run < d.list_cap ? run : d.list_cap);
-        if (run > d.list_cap) atomicOr(&ctl->error, 1);
+        ctl->count[1 - p] = total < d.list_cap ? total : d.list_cap;
+        if (total > d.list_cap) atomicOr(&ctl->error, 1);
         if (account) {
+            // Counters are read once and written once (every access is a dependent round trip
+            // for this single thread).
             const unsigned long long *v = sums.v;
             const unsigned long long sol = v[kPartClaimSol] + v[kPartNucSol] + v[kPartSettleSol];
-            StepStats *st = d.stats;
-            st->solidified += sol;
-            st->evaluated += v[kPartClaimEval];
-            st->masks += v[kPartClaimMask];
-            st->captures += v[kPartCap];
-            st->nucleated += v[kPartNucNn];
-            st->activated += v[kPartNucAct];
+            StepStats st = *d.stats;
+            const unsigned long long nonwaiting = ctl->nonwaiting + v[kPartClaimEntered] + v[kPartNucEntered];
+            const unsigned long long active = ctl->nactive + v[kPartNucAct];
+            st.solidified += sol;
+            st.evaluated += v[kPartClaimEval];
+            st.masks += v[kPartClaimMask];
+            st.captures += v[kPartCap];
+            st.nucleated += v[kPartNucNn];
+            st.activated += v[kPartNucAct];
             // The reference's numSteer of this step: active cells past their liquidus time at the
             // start of its capture pass.
-            ctl->nonwaiting += v[kPartClaimEntered] + v[kPartNucEntered];
-            const unsigned long long active = ctl->nactive + v[kPartNucAct];
-            st->frontier += ctl->nonwaiting;
-            st->waiting += active - ctl->nonwaiting;
-            ctl->nonwaiting -= sol;
+            st.frontier += nonwaiting;
+            st.waiting += active - nonwaiting;
+            *d.stats = st;
+            ctl->nonwaiting = nonwaiting - sol;
             ctl->nactive = active + v[kPartCap] - sol;
-            ctl->last_cycle = ctl->cycle[p];
-            maybe_stop(d, ctl, p);
+            const int cycle = ctl->cycle[p];
+            ctl->last_cycle = cycle;
+            if (d.autostop && cycle % 1000 == 0 && d.liquid0 - static_cast<long long>(st.captures) -
+                                                            static_cast<long long>(st.activated) == 0) {
+                ctl->done = 1;
+                ctl->final_cycle = cycle;
+                if (d.status) *d.status = cycle;
+            }
         }
     }
-    __syncthreads();
-    uint32_t run = part[threadIdx.x];
-    for (uint32_t c = s0; c < s1; c++) {
-        const uint32_t v = d.chunk_sum[c];
-        d.chunk_sum[c] = run;
-        run += v;
+    uint32_t run = base + incl - mine;
+    for (uint32_t c0b = s0; c0b < s1; c0b += 8) {
+        uint32_t v8[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) v8[k] = (c0b + k < s1) ? d.chunk_sum[c0b + k] : 0u;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            if (c0b + k < s1) d.chunk_sum[c0b + k] = run;
+            run += v8[k];
+        }
     }
 }
 
@@ -1258,7 +1316,7 @@ void launch_claim(const DomainView &d, int parity, int layer, int blocks, hipStr
 void launch_apply(const DomainView &d, int parity, int layer, int blocks, hipStream_t s) {
     const int cb = static_cast<int>(d.nchunks < static_cast<uint32_t>(blocks) ? d.nchunks : blocks);
     k_apply<<<apply_blocks(blocks), kBlock, 0, s>>>(d, parity, blocks);
-    k_settle<<<blocks, kBlock, 0, s>>>(d, parity, layer);
+    k_settle<<<settle_blocks(blocks), kBlock, 0, s>>>(d, parity, layer);
     k_bits_count<<<cb, kBlock, 0, s>>>(d, parity);
     k_bits_scan<<<1, kBlock, 0, s>>>(d, parity, 1, blocks, grid_for(d.nnuclei, 1024));
     k_bits_write<<<cb, kBlock, 0, s>>>(d, parity);
@@ -1284,7 +1342,7 @@ void launch_step_timed(const DomainView &d, int parity, int layer, int blocks, h
     k_apply<<<apply_blocks(blocks), kBlock, 0, s>>>(d, parity, blocks);
     rec();
     rec();
-    k_settle<<<blocks, kBlock, 0, s>>>(d, parity, layer);
+    k_settle<<<settle_blocks(blocks), kBlock, 0, s>>>(d, parity, layer);
     rec();
     rec();
     k_bits_count<<<cb, kBlock, 0, s>>>(d, parity);
@@ -1301,7 +1359,7 @@ void launch_step_n(const DomainView *views, int slots, int parity, int layer, in
     k_nucleate_n<<<dim3(nuc_blocks, slots), kBlock, 0, s>>>(views, parity, layer);
     k_claim_n<<<dim3(blocks, slots), kBlock, 0, s>>>(views, parity, layer);
     k_apply_n<<<dim3(apply_blocks(blocks), slots), kBlock, 0, s>>>(views, parity, blocks);
-    k_settle_n<<<dim3(blocks, slots), kBlock, 0, s>>>(views, parity, layer);
+    k_settle_n<<<dim3(settle_blocks(blocks), slots), kBlock, 0, s>>>(views, parity, layer);
     k_bits_count_n<<<dim3(blocks, slots), kBlock, 0, s>>>(views, parity);
     k_bits_scan_n<<<dim3(1, slots), kBlock, 0, s>>>(views, parity, blocks, nuc_blocks);
     k_bits_write_n<<<dim3(blocks, slots), kBlock, 0, s>>>(views, parity);

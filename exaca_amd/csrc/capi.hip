@@ -150,7 +150,8 @@ void build_device_state(exaca_state *h) {
     HIP_OK(hipGetDeviceProperties(&prop, dev));
     // Frontier kernels: up to 8 workgroups per CU, fewer for small domains (the frontier never
     // exceeds the cell count, and a slice is at least one workgroup of cells).
-    const long long cap = ((prop.multiProcessorCount * 8 + 7) / 8) * 8;
+    // (at most 2048: the per-workgroup step counters are summed by one workgroup, 8 slots a thread)
+    const long long cap = std::min<long long>(2048, ((prop.multiProcessorCount * 8 + 7) / 8) * 8);
     const long long want = ((hd.ncell / 4096 + 7) / 8) * 8;
     h->blocks = static_cast<int>(std::max<long long>(8, std::min(cap, want)));
 
@@ -197,7 +198,7 @@ void build_device_state(exaca_state *h) {
     h->check = h->alloc<CheckResult>(1);
     d.ctl = h->ctl;
     d.stats = h->stats;
-    d.nparts = static_cast<uint32_t>(std::max(h->blocks, 1024));
+    d.nparts = 2048;  // the capture grid (<= 2048) and the nucleation grid (<= 1024)
     d.part = h->alloc<unsigned long long>(static_cast<size_t>(kPartRows) * d.nparts);
 
     // Initial state; per-cell undercooling is not stored (it is a function of the steps executed
