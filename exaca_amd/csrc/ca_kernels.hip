@@ -315,6 +315,12 @@ __device__ __forceinline__ CellWords load_cell(const Cell *c) {
 __device__ __forceinline__ uint32_t *mask_word(const DomainView &d, uint32_t q) {
     return reinterpret_cast<uint32_t *>(&d.cells[q].lm);
 }
+// The high word: serial + 1 of the last step in which a capture by a cell of index >= q took one
+// of q's liquid neighbours (0: none yet). Written by the capture pass only where the low word is
+// known, so the 64-bit stores of a mask build (which clear it) never meet it in one step.
+__device__ __forceinline__ uint32_t *keep_word(const DomainView &d, uint32_t q) {
+    return reinterpret_cast<uint32_t *>(&d.cells[q].lm) + 1;
+}
 constexpr uint32_t kMaskUnknown = static_cast<uint32_t>(kLmUnknown);
 constexpr uint32_t kMaskNow = static_cast<uint32_t>(kLmNow);
 
@@ -707,7 +713,7 @@ __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t 
         // One capture per lane: item t belongs to the claimant s with pre[s] <= t < pre[s + 1].
         for (uint32_t t0 = 0; t0 < total; t0 += kBlock) {
             const uint32_t t = t0 + threadIdx.x;
-            uint32_t q = 0, emptied = 0;
+            uint32_t q = 0, emptied = 0, jc = 0;
             if (t < total) {
                 uint32_t lo = 0, hi = kBlock;  // pre[lo] <= t < pre[hi]
                 while (hi - lo > 1) {
@@ -717,7 +723,7 @@ __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t 
                     else
                         hi = mid;
                 }
-                const uint32_t jc = won_cell[lo];
+                jc = won_cell[lo];
                 uint32_t bits = won_bits[lo];
                 for (uint32_t r = t - pre[lo]; r; r--) bits &= bits - 1;
                 const int l = __builtin_ctz(bits);
@@ -762,6 +768,12 @@ __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t 
 #pragma unroll
                 for (int kk = 0; kk < 26; kk++)
                     if ((need >> kk) & 1u) mw[kk] = atomicAnd(mask_word(d, q + gr.off[kk]), ~(1u << kOpposite.v[kk]));
+                // a neighbour at or below the claimant's index still had q liquid at its visit
+                // in the reference's index order: its keep word gets this step's tag (every writer
+                // of a step writes the same value)
+#pragma unroll
+                for (int kk = 0; kk < 26; kk++)
+                    if (((need >> kk) & 1u) && q + gr.off[kk] <= jc) keep_word(d, q + gr.off[kk])[0] = serial + 1;
 #pragma unroll
                 for (int kk = 0; kk < 26; kk++) {
                     const uint32_t bit = 1u << kOpposite.v[kk];
@@ -838,10 +850,8 @@ __host__ __device__ constexpr uint32_t settle_blocks(uint32_t claim_blocks) {
 // order, so it grows and turns solid now (src/CAupdate.cpp:333-334, 722-725); a capture by itself
 // or by a later cell leaves it active, with no liquid neighbour: it turns solid at the next step. A
 // cell not past its liquidus time is not visited this step.
-// A candidate has no liquid neighbour left, so its claim-time liquid neighbours are exactly its
-// neighbours captured in this step: the active cells whose liquid mask is not yet known (every
-// cell captured before this step built its mask at its first capture pass, which came before this
-// step's captures). Their claim words still name their winners.
+// A candidate's mask was emptied by this step's captures, each of which tagged the candidate's
+// keep word with this step when its claimant's index is >= the candidate's: the tag decides.
 __device__ __forceinline__ void settle_body(const DomainView &d, int p, int layer, uint32_t b, uint32_t nb) {
     __shared__ unsigned long long sh[kBlock / 64];
     StepCtl *ctl = d.ctl;
@@ -850,7 +860,6 @@ __device__ __forceinline__ void settle_body(const DomainView &d, int p, int laye
     const uint32_t serial = ctl->serial[p];
     const uint32_t n = ctl->ncand < d.list_cap ? ctl->ncand : d.list_cap;
     const uint32_t *cand = d.list[1 - p];
-    const Grid gr = grid_of(d);
     StepCounts c{0, 0, 0, 0};
     __shared__ double gtab[kGrowStage];
     const Growth g = staged_growth(d, gtab);
@@ -859,24 +868,7 @@ __device__ __forceinline__ void settle_body(const DomainView &d, int p, int laye
         const CellWords w = load_cell(d.cells + r);
         if ((w.ct & kTypeMask) != kActive) continue;
         if (!(cycle > w.cts && w.lid <= layer)) continue;
-        const uint32_t valid = valid_neighbours(decode(r, gr), gr.X, gr.Y, gr.nzA);
-        // the 26 neighbour records loaded together (a neighbour outside the domain loads r itself
-        // and is masked out), then tested with branch-free arithmetic
-        uint32_t lo[26], win[26], tw[26];
-#pragma unroll
-        for (int l = 0; l < 26; l++) {
-            const uint32_t q = ((valid >> l) & 1u) ? r + gr.off[l] : r;
-            lo[l] = *mask_word(d, q);
-            const uint4 hi = reinterpret_cast<const uint4 *>(d.cells + q)[1];  // cts, gid, win, lid|ct|flags
-            win[l] = hi.z;
-            tw[l] = hi.w;
-        }
-        uint32_t keep = 0;
-#pragma unroll
-        for (int l = 0; l < 26; l++) {
-            const uint32_t captured_now = (lo[l] >> 31) & static_cast<uint32_t>(((tw[l] >> 8) & kTypeMask) == kActive);
-            keep |= ((valid >> l) & 1u) & captured_now & static_cast<uint32_t>(win[l] >= r);
-        }
+        const bool keep = static_cast<uint32_t>(w.lm >> 32) == serial + 1;
         if (!keep) {
             solidify(d, g, r, w, growth_index(d, w.cts, cycle));
             c.nsol++;
@@ -968,6 +960,7 @@ __device__ __forceinline__ StepSums sum_parts(const DomainView &d, uint32_t nb, 
 // liquid state by capture or nucleation, so the reference's superheated + undercooled count
 // (src/CAupdate.cpp:738-783) is the initial liquid count minus both); a rebuild between steps
 // passes 0.
+constexpr uint32_t kScanRegs = 32;
 __device__ __forceinline__ void bits_scan_body(const DomainView &d, int p, int account, uint32_t nb, uint32_t nnb) {
     __shared__ uint32_t part[kBlock / 64];
     StepCtl *ctl = d.ctl;
@@ -980,13 +973,24 @@ __device__ __forceinline__ void bits_scan_body(const DomainView &d, int p, int a
     const uint32_t per = (nc + kBlock - 1) / kBlock;
     const uint32_t s0 = c0 + threadIdx.x * per;
     const uint32_t s1 = any ? (s0 + per <= c1 + 1 ? s0 + per : c1 + 1) : 0;
+    // up to kScanRegs chunks per thread are loaded at once and kept in registers for the write
+    // (one round trip each way); longer segments go round in batches
+    const bool inreg = per <= kScanRegs;
+    uint32_t v[kScanRegs];
     uint32_t mine = 0;
-    for (uint32_t c0b = s0; c0b < s1; c0b += 8) {
-        uint32_t v8[8];
+    if (inreg) {
 #pragma unroll
-        for (int k = 0; k < 8; k++) v8[k] = (c0b + k < s1) ? d.chunk_sum[c0b + k] : 0u;
+        for (uint32_t k = 0; k < kScanRegs; k++) v[k] = (s0 + k < s1) ? d.chunk_sum[s0 + k] : 0u;
 #pragma unroll
-        for (int k = 0; k < 8; k++) mine += v8[k];
+        for (uint32_t k = 0; k < kScanRegs; k++) mine += v[k];
+    }
+    else {
+        for (uint32_t c0b = s0; c0b < s1; c0b += kScanRegs) {
+#pragma unroll
+            for (uint32_t k = 0; k < kScanRegs; k++) v[k] = (c0b + k < s1) ? d.chunk_sum[c0b + k] : 0u;
+#pragma unroll
+            for (uint32_t k = 0; k < kScanRegs; k++) mine += v[k];
+        }
     }
     // exclusive scan of the threads' segment sums (wave scans, then the wave totals)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1034,14 +1038,21 @@ __device__ __forceinline__ void bits_scan_body(const DomainView &d, int p, int a
         }
     }
     uint32_t run = base + incl - mine;
-    for (uint32_t c0b = s0; c0b < s1; c0b += 8) {
-        uint32_t v8[8];
+    if (inreg) {
 #pragma unroll
-        for (int k = 0; k < 8; k++) v8[k] = (c0b + k < s1) ? d.chunk_sum[c0b + k] : 0u;
+        for (uint32_t k = 0; k < kScanRegs; k++) {
+            if (s0 + k < s1) d.chunk_sum[s0 + k] = run;
+            run += v[k];
+        }
+        return;
+    }
+    for (uint32_t c0b = s0; c0b < s1; c0b += kScanRegs) {
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
+        for (uint32_t k = 0; k < kScanRegs; k++) v[k] = (c0b + k < s1) ? d.chunk_sum[c0b + k] : 0u;
+#pragma unroll
+        for (uint32_t k = 0; k < kScanRegs; k++) {
             if (c0b + k < s1) d.chunk_sum[c0b + k] = run;
-            run += v8[k];
+            run += v[k];
         }
     }
 }
