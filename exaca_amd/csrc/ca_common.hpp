@@ -5,7 +5,8 @@
 // cells are sparse in that order (a front of a few percent of each plane over a band of planes),
 // so everything a step reads or atomically updates for a cell sits in one aligned 32-byte sector:
 //   lm   uint64  liquid neighbours of an active cell. Bits 0-25: now; bit 31 set: not yet known
-//                (a new cell, or never active).
+//                (a new cell, or never active). Bits 32-63: serial + 1 of the last step in which
+//                a cell of index >= this one captured one of its liquid neighbours (keep tag).
 //   dl   float   octahedron half-diagonal (DiagonalLength) after the growth of steps up to `win`
 //   thr  float   lower bound on the critical lengths of the still-liquid neighbours (0: unknown)
 //   cts  int32   CritTimeStep (liquidus crossing step)
@@ -24,6 +25,7 @@
 // recomputed bit-exactly from the double centre (captured cells) or read from a per-orientation
 // table (cell-centred octahedra: nuclei and the initial substrate).
 #pragma once
+#include <cstddef>
 #include <cstdint>
 
 namespace exaca {
@@ -48,6 +50,11 @@ constexpr uint16_t kCounted = 1;
 // Steps ahead a cell can be scheduled; a cell with nothing to do for longer is scheduled at the
 // horizon and checks again.
 constexpr uint32_t kAlarmSlots = 64;
+// Words of one calendar slot: one bit per cell, rounded up to 32-byte rows (the list rebuild reads
+// a slot in 32-byte pieces).
+constexpr size_t alarm_stride(long long ncell) {
+    return ((static_cast<size_t>(ncell) + 31) / 32 + 7) & ~static_cast<size_t>(7);
+}
 constexpr int kMaxJumps = 4096;
 
 // The 26 neighbour offsets in the reference order: 0-8 at dy = -1, 9-16 in the y plane, 17-25 at
@@ -91,6 +98,7 @@ struct StepCtl {
     uint32_t count[2];    // lengths of the steps' lists of cells to process
     uint32_t elo, ehi;    // cell-index range [elo, ehi] holding every active cell (elo > ehi: none)
     uint32_t ncand;       // cells whose last liquid neighbour was captured in the current step
+    uint32_t rebuild_seq; // advanced before every list rebuild (its flags' tag)
     int32_t njumps;       // cycle jumps taken so far (spot-melt problems)
     int32_t error;        // nonzero: list overflow or invalid grain
     int32_t done;         // set by the device when a domain with autostop finishes
@@ -194,7 +202,7 @@ struct DomainView {
     uint32_t nparts;
     uint32_t *alarms;       // kAlarmSlots bitmaps of one bit per cell: process the cell at the step
                             // whose serial is the bitmap's slot (mod kAlarmSlots)
-    uint32_t *chunk_sum;    // set bits per 2048-word chunk of a bitmap, then their exclusive prefix
+    unsigned long long *chunk_flag;  // per unit of the list rebuild: tag | state | count or prefix
     uint32_t nchunks;
     int32_t autostop;       // end the domain on the device at the first check with no liquid left
     long long liquid0;      // liquid cells of the layer at the start

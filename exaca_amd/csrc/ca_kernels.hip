@@ -1,6 +1,6 @@
 // ca_kernels.hip -- the CA time step as memory-bound CDNA4 kernels over the cells due this step.
 //
-// One step (reference: src/runCA.cpp:291-311) is seven launches on one stream:
+// One step (reference: src/runCA.cpp:291-311) is five launches on one stream:
 //   k_nucleate  activates the pre-sampled nuclei whose time has come (src/CAupdate.cpp:67-200)
 //   k_claim     for every cell due this step: the growth since it was last looked at, a claim on
 //               every liquid neighbour it reaches (atomicMin of its index into the neighbour's
@@ -9,7 +9,7 @@
 //   k_apply     the winner of each claimed cell builds the new decentred octahedron and takes the
 //               cell out of its neighbours' liquid masks
 //   k_settle    a cell whose liquid neighbours were all taken by lower-indexed cells turns solid
-//   k_bits_*    the list of the cells due at the next step, in index order, from its calendar slot
+//   k_rebuild   the list of the cells due at the next step, in index order, from its calendar slot
 // A cell is due when something can happen to it: it is new, its liquidus time has passed, or its
 // half-diagonal reaches the least critical length of its liquid neighbours; a neighbour's capture
 // or nucleation that empties its mask is handled where it happens. Between those steps its growth
@@ -383,7 +383,7 @@ __device__ uint32_t steps_until(const Growth &g, float dl, int k, float thr) {
 // kAlarmSlots is set. ----
 __device__ __forceinline__ size_t bitmap_words(const DomainView &d) { return static_cast<size_t>((d.ncell + 31) / 32); }
 __device__ __forceinline__ void set_alarm(const DomainView &d, uint32_t serial, uint32_t q) {
-    atomicOr(&d.alarms[(serial % kAlarmSlots) * bitmap_words(d) + (q >> 5)], 1u << (q & 31));
+    atomicOr(&d.alarms[(serial % kAlarmSlots) * alarm_stride(d.ncell) + (q >> 5)], 1u << (q & 31));
 }
 
 struct StepCounts {
@@ -421,6 +421,7 @@ __device__ __forceinline__ void nucleate_body(const DomainView &d, int p, int la
         ctl->cycle[1 - p] = cycle + 1;
         ctl->serial[1 - p] = ctl->serial[p] + 1;
         ctl->ncand = 0;
+        ctl->rebuild_seq = ctl->rebuild_seq + 1;
     }
     const Grid gr = grid_of(d);
     unsigned long long nn = 0, act = 0;
@@ -882,10 +883,24 @@ __device__ __forceinline__ void settle_body(const DomainView &d, int p, int laye
 
 
 // The list of the next step is every set bit of its calendar slot, in ascending cell index, over
-// the chunks of the range of active cells; the slot is cleared as it is read. Three passes: per-chunk
-// counts, one exclusive scan of the chunk counts, and the write.
-constexpr uint32_t kChunkWords = 8 * kBlock;
+// the chunks of the range of active cells; the slot is cleared as it is read. One pass: workgroup
+// b of nbw - 1 takes units b, b + nbw - 1, ... of kUnitChunks chunks, counts a unit's bits, and
+// finds the unit's offset by decoupled look-back over the units before it (each unit publishes its
+// count, then its inclusive prefix, as one tagged 64-bit word); then it writes the unit's cells.
+// A unit's count never waits on anything, and a unit waits only for lower units, of workgroups
+// dispatched before it or of earlier rounds of workgroups that are resident with it (the plain
+// launch has at most 257 workgroups); each wait is bounded all the same (error 8). The last
+// workgroup books the step meanwhile. (The range is a few thousand chunks at 800^3: a few hundred
+// units, about one per CU.)
+constexpr uint32_t kChunkWords = 8 * kBlock;  // thread t of a chunk holds its words 8t .. 8t + 7
 constexpr uint32_t kChunkCells = 32 * kChunkWords;
+constexpr uint32_t kUnitChunks = 8;
+constexpr uint32_t kRebuildStage = 8192;  // list entries of a unit staged in LDS (32 KB)
+constexpr unsigned long long kFlagCount = 1ull << 32;   // value = the chunk's own count
+constexpr unsigned long long kFlagPrefix = 2ull << 32;  // value = count of the chunk and all before it
+constexpr unsigned long long kFlagState = 3ull << 32;
+constexpr unsigned long long kFlagTag = ~((1ull << 34) - 1);
+constexpr uint32_t kLookBackSpins = 1u << 20;  // bound on the waits of one look-back (error 8 past it)
 
 __device__ __forceinline__ bool chunk_range(const DomainView &d, uint32_t &c0, uint32_t &c1) {
     const uint32_t lo = d.ctl->elo;
@@ -897,25 +912,49 @@ __device__ __forceinline__ bool chunk_range(const DomainView &d, uint32_t &c0, u
     return true;
 }
 __device__ __forceinline__ uint32_t *next_slot(const DomainView &d, int p) {
-    return d.alarms + ((d.ctl->serial[p] + 1) % kAlarmSlots) * bitmap_words(d);
+    return d.alarms + ((d.ctl->serial[p] + 1) % kAlarmSlots) * alarm_stride(d.ncell);
+}
+// Flags of one rebuild carry its tag (the rebuild sequence number, advanced by the kernels that
+// precede a rebuild: nucleation and rewake; tags cycle through 2^30 - 1 nonzero values), so the
+// flags of earlier rebuilds need no clearing.
+__device__ __forceinline__ unsigned long long flag_tag(uint32_t seq) {
+    return static_cast<unsigned long long>(seq % ((1u << 30) - 1) + 1) << 34;
+}
+__device__ __forceinline__ void publish(unsigned long long *f, unsigned long long v) {
+    __hip_atomic_store(f, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long peek(unsigned long long *f) {
+    return __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ __forceinline__ void bits_count_body(const DomainView &d, int p, uint32_t b, uint32_t nb) {
-    __shared__ unsigned long long sh[kBlock / 64];
-    if (d.ctl->done) return;
-    uint32_t c0, c1;
-    if (!chunk_range(d, c0, c1)) return;
-    const uint32_t nwords = static_cast<uint32_t>(bitmap_words(d));
-    const uint32_t *bits = next_slot(d, p);
-    for (uint32_t c = c0 + b; c <= c1; c += nb) {
-        uint32_t cnt = 0;
-#pragma unroll
-        for (uint32_t k = 0; k < kChunkWords / kBlock; k++) {
-            const uint32_t wi = c * kChunkWords + k * kBlock + threadIdx.x;
-            if (wi < nwords) cnt += __builtin_popcount(bits[wi]);
+// Cells before unit u (u > 0), by one wave: windows of 64 predecessors, nearest first, summed up
+// to the nearest one holding a prefix; unit 0 always holds one.
+__device__ uint32_t look_back(const DomainView &d, uint32_t u, unsigned long long tag) {
+    const int lane = threadIdx.x & 63;
+    uint32_t excl = 0, spins = 0;
+    long long j0 = static_cast<long long>(u) - 1;
+    for (;;) {
+        const long long j = j0 - lane;
+        const unsigned long long f = j >= 0 ? peek(d.chunk_flag + j) : (tag | kFlagPrefix);
+        const bool valid = (f & kFlagTag) == tag && (f & kFlagState) != 0;
+        const unsigned long long pm = __ballot((f & kFlagTag) == tag && (f & kFlagState) == kFlagPrefix);
+        const unsigned long long vm = __ballot(valid);
+        const int np = pm ? __builtin_ctzll(pm) : 63;
+        const unsigned long long need = np >= 63 ? ~0ull : ((2ull << np) - 1);
+        if ((vm & need) != need) {
+            if (++spins > kLookBackSpins) {
+                if (lane == 0) atomicOr(&d.ctl->error, 8);
+                return excl;
+            }
+            __builtin_amdgcn_s_sleep(2);
+            continue;
         }
-        const unsigned long long t = block_sum(cnt, sh);
-        if (threadIdx.x == 0) d.chunk_sum[c] = static_cast<uint32_t>(t);
+        uint32_t v = lane <= np ? static_cast<uint32_t>(f) : 0u;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        excl += v;
+        if (pm) return excl;
+        j0 -= 64;
     }
 }
 
@@ -955,151 +994,169 @@ __device__ __forceinline__ StepSums sum_parts(const DomainView &d, uint32_t nb, 
     return s;
 }
 
-// account: this closes the step (its counters are booked, and a domain with autostop ends at a
-// check step (cycle a multiple of 1000) with no liquid left: liquid cells only ever leave the
-// liquid state by capture or nucleation, so the reference's superheated + undercooled count
-// (src/CAupdate.cpp:738-783) is the initial liquid count minus both); a rebuild between steps
-// passes 0.
-constexpr uint32_t kScanRegs = 32;
-__device__ __forceinline__ void bits_scan_body(const DomainView &d, int p, int account, uint32_t nb, uint32_t nnb) {
-    __shared__ uint32_t part[kBlock / 64];
+// Closing a rebuild (the list's length is set by the last unit, or here when there is none): with
+// account the step's counters are booked and a domain with autostop ends at a check step (cycle a
+// multiple of 1000) with no liquid left: liquid cells only ever leave the liquid state by capture
+// or nucleation, so the reference's superheated + undercooled count (src/CAupdate.cpp:738-783) is
+// the initial liquid count minus both. A rebuild between steps passes account 0.
+__device__ __forceinline__ void set_count(const DomainView &d, int p, uint32_t total) {
+    d.ctl->count[1 - p] = total < d.list_cap ? total : d.list_cap;
+    if (total > d.list_cap) atomicOr(&d.ctl->error, 1);
+}
+__device__ __forceinline__ void close_rebuild(const DomainView &d, int p, int account, uint32_t nb, uint32_t nnb,
+                                              uint32_t nunits) {
     StepCtl *ctl = d.ctl;
-    if (ctl->done) return;
     StepSums sums;
     if (account) sums = sum_parts(d, nb, nnb);
-    uint32_t c0 = 1, c1 = 0;
-    const bool any = chunk_range(d, c0, c1);
-    const uint32_t nc = any ? c1 - c0 + 1 : 0;
-    const uint32_t per = (nc + kBlock - 1) / kBlock;
-    const uint32_t s0 = c0 + threadIdx.x * per;
-    const uint32_t s1 = any ? (s0 + per <= c1 + 1 ? s0 + per : c1 + 1) : 0;
-    // up to kScanRegs chunks per thread are loaded at once and kept in registers for the write
-    // (one round trip each way); longer segments go round in batches
-    const bool inreg = per <= kScanRegs;
-    uint32_t v[kScanRegs];
-    uint32_t mine = 0;
-    if (inreg) {
-#pragma unroll
-        for (uint32_t k = 0; k < kScanRegs; k++) v[k] = (s0 + k < s1) ? d.chunk_sum[s0 + k] : 0u;
-#pragma unroll
-        for (uint32_t k = 0; k < kScanRegs; k++) mine += v[k];
-    }
-    else {
-        for (uint32_t c0b = s0; c0b < s1; c0b += kScanRegs) {
-#pragma unroll
-            for (uint32_t k = 0; k < kScanRegs; k++) v[k] = (c0b + k < s1) ? d.chunk_sum[c0b + k] : 0u;
-#pragma unroll
-            for (uint32_t k = 0; k < kScanRegs; k++) mine += v[k];
-        }
-    }
-    // exclusive scan of the threads' segment sums (wave scans, then the wave totals)
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t wtot;
-    const uint32_t incl = wave_inclusive_scan(mine, wtot);
-    if (lane == 63) part[wave] = wtot;
-    __syncthreads();
-    uint32_t base = 0, total = 0;
-    for (int w = 0; w < kBlock / 64; w++) {
-        if (w < wave) base += part[w];
-        total += part[w];
-    }
-    if (threadIdx.x == 0) {
-        ctl->count[1 - p] = total < d.list_cap ? total : d.list_cap;
-        if (total > d.list_cap) atomicOr(&ctl->error, 1);
-        if (account) {
-            // Counters are read once and written once (every access is a dependent round trip
-            // for this single thread).
-            const unsigned long long *v = sums.v;
-            const unsigned long long sol = v[kPartClaimSol] + v[kPartNucSol] + v[kPartSettleSol];
-            StepStats st = *d.stats;
-            const unsigned long long nonwaiting = ctl->nonwaiting + v[kPartClaimEntered] + v[kPartNucEntered];
-            const unsigned long long active = ctl->nactive + v[kPartNucAct];
-            st.solidified += sol;
-            st.evaluated += v[kPartClaimEval];
-            st.masks += v[kPartClaimMask];
-            st.captures += v[kPartCap];
-            st.nucleated += v[kPartNucNn];
-            st.activated += v[kPartNucAct];
-            // The reference's numSteer of this step: active cells past their liquidus time at the
-            // start of its capture pass.
-            st.frontier += nonwaiting;
-            st.waiting += active - nonwaiting;
-            *d.stats = st;
-            ctl->nonwaiting = nonwaiting - sol;
-            ctl->nactive = active + v[kPartCap] - sol;
-            const int cycle = ctl->cycle[p];
-            ctl->last_cycle = cycle;
-            if (d.autostop && cycle % 1000 == 0 && d.liquid0 - static_cast<long long>(st.captures) -
-                                                            static_cast<long long>(st.activated) == 0) {
-                ctl->done = 1;
-                ctl->final_cycle = cycle;
-                if (d.status) *d.status = cycle;
-            }
-        }
-    }
-    uint32_t run = base + incl - mine;
-    if (inreg) {
-#pragma unroll
-        for (uint32_t k = 0; k < kScanRegs; k++) {
-            if (s0 + k < s1) d.chunk_sum[s0 + k] = run;
-            run += v[k];
-        }
-        return;
-    }
-    for (uint32_t c0b = s0; c0b < s1; c0b += kScanRegs) {
-#pragma unroll
-        for (uint32_t k = 0; k < kScanRegs; k++) v[k] = (c0b + k < s1) ? d.chunk_sum[c0b + k] : 0u;
-#pragma unroll
-        for (uint32_t k = 0; k < kScanRegs; k++) {
-            if (c0b + k < s1) d.chunk_sum[c0b + k] = run;
-            run += v[k];
-        }
+    if (threadIdx.x != 0) return;
+    if (!nunits) set_count(d, p, 0);
+    if (!account) return;
+    // Counters are read once and written once (every access is a dependent round trip for this
+    // single thread).
+    const unsigned long long *v = sums.v;
+    const unsigned long long sol = v[kPartClaimSol] + v[kPartNucSol] + v[kPartSettleSol];
+    StepStats st = *d.stats;
+    const unsigned long long nonwaiting = ctl->nonwaiting + v[kPartClaimEntered] + v[kPartNucEntered];
+    const unsigned long long active = ctl->nactive + v[kPartNucAct];
+    st.solidified += sol;
+    st.evaluated += v[kPartClaimEval];
+    st.masks += v[kPartClaimMask];
+    st.captures += v[kPartCap];
+    st.nucleated += v[kPartNucNn];
+    st.activated += v[kPartNucAct];
+    // The reference's numSteer of this step: active cells past their liquidus time at the start of
+    // its capture pass.
+    st.frontier += nonwaiting;
+    st.waiting += active - nonwaiting;
+    *d.stats = st;
+    ctl->nonwaiting = nonwaiting - sol;
+    ctl->nactive = active + v[kPartCap] - sol;
+    const int cycle = ctl->cycle[p];
+    ctl->last_cycle = cycle;
+    if (d.autostop && cycle % 1000 == 0 &&
+        d.liquid0 - static_cast<long long>(st.captures) - static_cast<long long>(st.activated) == 0) {
+        ctl->done = 1;
+        ctl->final_cycle = cycle;
+        if (d.status) *d.status = cycle;
     }
 }
 
-// Chunk c is written in kChunkWords / kBlock rounds of one bitmap word per thread (consecutive
-// threads, consecutive words). A round's cells are staged in LDS at their offsets within the round
-// and then stored to the list with consecutive lanes on consecutive entries.
-__device__ __forceinline__ void bits_write_body(const DomainView &d, int p, uint32_t b, uint32_t nb) {
-    __shared__ uint32_t wave_tot[kBlock / 64];
-    __shared__ uint32_t stage[32 * kBlock];
+__device__ __forceinline__ void rebuild_body(const DomainView &d, int p, int account, uint32_t nb, uint32_t nnb,
+                                             uint32_t b, uint32_t nbw) {
+    __shared__ uint32_t sh_excl;
+    __shared__ uint32_t wave_tot[kUnitChunks][kBlock / 64];
+    __shared__ uint32_t stage[kRebuildStage];
     StepCtl *ctl = d.ctl;
     if (ctl->done) return;
-    uint32_t c0, c1;
-    if (!chunk_range(d, c0, c1)) return;
+    uint32_t c0 = 1, c1 = 0;
+    const bool any = chunk_range(d, c0, c1);
+    const uint32_t nunits = any ? (c1 - c0) / kUnitChunks + 1 : 0;
+    if (b == nbw - 1) {
+        close_rebuild(d, p, account, nb, nnb, nunits);
+        return;
+    }
+    const unsigned long long tag = flag_tag(ctl->rebuild_seq);
     const uint32_t nwords = static_cast<uint32_t>(bitmap_words(d));
     uint32_t *bits = next_slot(d, p);
     uint32_t *next = d.list[1 - p];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (uint32_t c = c0 + b; c <= c1; c += nb) {
-        uint32_t base = d.chunk_sum[c];
-        for (uint32_t r = 0; r < kChunkWords / kBlock; r++) {
-            const uint32_t wi = c * kChunkWords + r * kBlock + threadIdx.x;
-            uint32_t w = 0;
-            if (wi < nwords) {
-                w = bits[wi];
-                if (w) bits[wi] = 0;
+    for (uint32_t t = b; t < nunits; t += nbw - 1) {
+        // the unit's words, all loads in flight together
+        uint32_t w[kUnitChunks][8];
+#pragma unroll
+        for (uint32_t k = 0; k < kUnitChunks; k++) {
+            const uint32_t c = c0 + t * kUnitChunks + k;
+            const uint32_t w0 = c * kChunkWords + 8 * threadIdx.x;
+            if (c <= c1 && w0 + 8 <= nwords) {
+                const uint4 a = reinterpret_cast<const uint4 *>(bits + w0)[0];
+                const uint4 b = reinterpret_cast<const uint4 *>(bits + w0)[1];
+                w[k][0] = a.x, w[k][1] = a.y, w[k][2] = a.z, w[k][3] = a.w;
+                w[k][4] = b.x, w[k][5] = b.y, w[k][6] = b.z, w[k][7] = b.w;
             }
-            const uint32_t cnt = __builtin_popcount(w);
-            uint32_t wtot;
-            const uint32_t incl = wave_inclusive_scan(cnt, wtot);
-            if (lane == 63) wave_tot[wave] = wtot;
-            __syncthreads();
-            uint32_t o = 0, rtot = 0;
-            for (int q = 0; q < kBlock / 64; q++) {
-                if (q < wave) o += wave_tot[q];
-                rtot += wave_tot[q];
+            else {
+#pragma unroll
+                for (int i = 0; i < 8; i++) w[k][i] = (c <= c1 && w0 + i < nwords) ? bits[w0 + i] : 0u;
             }
-            o += incl - cnt;
-            for (uint32_t m = w; m; m &= m - 1) stage[o++] = wi * 32u + __builtin_ctz(m);
-            __syncthreads();
-            for (uint32_t k = threadIdx.x; k < rtot; k += kBlock) {
-                const uint32_t pos = base + k;
-                if (pos < d.list_cap) next[pos] = stage[k];
-            }
-            base += rtot;
-            __syncthreads();
         }
+        uint32_t cnt[kUnitChunks], incl[kUnitChunks];
+#pragma unroll
+        for (uint32_t k = 0; k < kUnitChunks; k++) {
+            cnt[k] = 0;
+#pragma unroll
+            for (int i = 0; i < 8; i++) cnt[k] += __builtin_popcount(w[k][i]);
+            uint32_t wtot;
+            incl[k] = wave_inclusive_scan(cnt[k], wtot);
+            if (lane == 63) wave_tot[k][wave] = wtot;
+        }
+        __syncthreads();
+        // offset of this thread's cells within the unit (chunk by chunk, thread by thread)
+        uint32_t off[kUnitChunks], agg = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kUnitChunks; k++) {
+            uint32_t before = 0, tot = 0;
+            for (int q = 0; q < kBlock / 64; q++) {
+                if (q < wave) before += wave_tot[k][q];
+                tot += wave_tot[k][q];
+            }
+            off[k] = agg + before + incl[k] - cnt[k];
+            agg += tot;
+        }
+        if (wave == 0) {
+            uint32_t excl = 0;
+            if (t == 0) {
+                if (lane == 0) publish(d.chunk_flag + t, tag | kFlagPrefix | agg);
+            }
+            else {
+                if (lane == 0) publish(d.chunk_flag + t, tag | kFlagCount | agg);
+                excl = look_back(d, t, tag);
+                if (lane == 0) publish(d.chunk_flag + t, tag | kFlagPrefix | (excl + agg));
+            }
+            if (lane == 0) {
+                sh_excl = excl;
+                // the last unit knows the list's length
+                if (t == nunits - 1) set_count(d, p, excl + agg);
+            }
+        }
+        // the slot is cleared where it held bits; the cells go to LDS in list order when they fit
+        const bool staged = agg <= kRebuildStage;
+#pragma unroll
+        for (uint32_t k = 0; k < kUnitChunks; k++) {
+            const uint32_t c = c0 + t * kUnitChunks + k;
+            const uint32_t w0 = c * kChunkWords + 8 * threadIdx.x;
+            if (c <= c1 && w0 + 8 <= nwords) {
+                if (w[k][0] | w[k][1] | w[k][2] | w[k][3]) reinterpret_cast<uint4 *>(bits + w0)[0] = make_uint4(0, 0, 0, 0);
+                if (w[k][4] | w[k][5] | w[k][6] | w[k][7]) reinterpret_cast<uint4 *>(bits + w0)[1] = make_uint4(0, 0, 0, 0);
+            }
+            else {
+#pragma unroll
+                for (int i = 0; i < 8; i++)
+                    if (w[k][i]) bits[w0 + i] = 0u;
+            }
+            if (staged) {
+                uint32_t o = off[k];
+#pragma unroll
+                for (int i = 0; i < 8; i++)
+                    for (uint32_t m = w[k][i]; m; m &= m - 1) stage[o++] = (w0 + i) * 32u + __builtin_ctz(m);
+            }
+        }
+        __syncthreads();
+        const uint32_t excl = sh_excl;
+        if (staged) {
+            for (uint32_t i = threadIdx.x; i < agg; i += kBlock)
+                if (excl + i < d.list_cap) next[excl + i] = stage[i];
+        }
+        else {
+#pragma unroll
+            for (uint32_t k = 0; k < kUnitChunks; k++) {
+                const uint32_t w0 = (c0 + t * kUnitChunks + k) * kChunkWords + 8 * threadIdx.x;
+                uint32_t pos = excl + off[k];
+#pragma unroll
+                for (int i = 0; i < 8; i++)
+                    for (uint32_t m = w[k][i]; m; m &= m - 1, pos++)
+                        if (pos < d.list_cap) next[pos] = (w0 + i) * 32u + __builtin_ctz(m);
+            }
+        }
+        __syncthreads();  // the stage and the scan words are reused by the next unit
     }
 }
 
@@ -1107,6 +1164,7 @@ __device__ __forceinline__ void bits_write_body(const DomainView &d, int p, uint
 // not count).
 __device__ __forceinline__ void rewake_body(const DomainView &d, int p_next, uint32_t b, uint32_t nb) {
     StepCtl *ctl = d.ctl;
+    if (b == 0 && threadIdx.x == 0) ctl->rebuild_seq = ctl->rebuild_seq + 1;
     const uint32_t lo = ctl->elo, hi = ctl->ehi;
     if (lo > hi) return;
     const uint32_t serial = ctl->serial[p_next];
@@ -1128,14 +1186,8 @@ __global__ __launch_bounds__(kBlock) void k_apply(DomainView d, int p, int nbc) 
 __global__ __launch_bounds__(kBlock) void k_settle(DomainView d, int p, int layer) {
     settle_body(d, p, layer, blockIdx.x, gridDim.x);
 }
-__global__ __launch_bounds__(kBlock) void k_bits_count(DomainView d, int p) {
-    bits_count_body(d, p, blockIdx.x, gridDim.x);
-}
-__global__ __launch_bounds__(kBlock) void k_bits_scan(DomainView d, int p, int account, int nb, int nnb) {
-    bits_scan_body(d, p, account, static_cast<uint32_t>(nb), static_cast<uint32_t>(nnb));
-}
-__global__ __launch_bounds__(kBlock) void k_bits_write(DomainView d, int p) {
-    bits_write_body(d, p, blockIdx.x, gridDim.x);
+__global__ __launch_bounds__(kBlock) void k_rebuild(DomainView d, int p, int account, int nb, int nnb) {
+    rebuild_body(d, p, account, static_cast<uint32_t>(nb), static_cast<uint32_t>(nnb), blockIdx.x, gridDim.x);
 }
 __global__ __launch_bounds__(kBlock) void k_rewake(DomainView d, int p_next) {
     rewake_body(d, p_next, blockIdx.x, gridDim.x);
@@ -1159,17 +1211,9 @@ __global__ __launch_bounds__(kBlock) void k_settle_n(const DomainView *v, int p,
     const DomainView &d = v[blockIdx.y];
     if (d.ctl) settle_body(d, p, layer, blockIdx.x, gridDim.x);
 }
-__global__ __launch_bounds__(kBlock) void k_bits_count_n(const DomainView *v, int p) {
+__global__ __launch_bounds__(kBlock) void k_rebuild_n(const DomainView *v, int p, int nb, int nnb) {
     const DomainView &d = v[blockIdx.y];
-    if (d.ctl) bits_count_body(d, p, blockIdx.x, gridDim.x);
-}
-__global__ __launch_bounds__(kBlock) void k_bits_scan_n(const DomainView *v, int p, int nb, int nnb) {
-    const DomainView &d = v[blockIdx.y];
-    if (d.ctl) bits_scan_body(d, p, 1, static_cast<uint32_t>(nb), static_cast<uint32_t>(nnb));
-}
-__global__ __launch_bounds__(kBlock) void k_bits_write_n(const DomainView *v, int p) {
-    const DomainView &d = v[blockIdx.y];
-    if (d.ctl) bits_write_body(d, p, blockIdx.x, gridDim.x);
+    if (d.ctl) rebuild_body(d, p, 1, static_cast<uint32_t>(nb), static_cast<uint32_t>(nnb), blockIdx.x, gridDim.x);
 }
 
 __global__ __launch_bounds__(kBlock) void k_check(DomainView d, int cycle, int layer, CheckResult *out) {
@@ -1324,23 +1368,26 @@ void launch_nucleation(const DomainView &d, int parity, int layer, hipStream_t s
 void launch_claim(const DomainView &d, int parity, int layer, int blocks, hipStream_t s) {
     k_claim<<<blocks, kBlock, 0, s>>>(d, parity, layer);
 }
+// Workgroups of the list rebuild: at most one per unit of the whole domain and one per CU (each
+// takes every (n-1)-th unit), plus the one that books the step.
+int rebuild_blocks(const DomainView &d, int blocks) {
+    const uint32_t units = (d.nchunks + kUnitChunks - 1) / kUnitChunks;
+    const uint32_t cap = static_cast<uint32_t>(blocks < 256 ? blocks : 256);
+    return static_cast<int>(units < cap ? units : cap) + 1;
+}
 void launch_apply(const DomainView &d, int parity, int layer, int blocks, hipStream_t s) {
-    const int cb = static_cast<int>(d.nchunks < static_cast<uint32_t>(blocks) ? d.nchunks : blocks);
+    const int cb = rebuild_blocks(d, blocks);
     k_apply<<<apply_blocks(blocks), kBlock, 0, s>>>(d, parity, blocks);
     k_settle<<<settle_blocks(blocks), kBlock, 0, s>>>(d, parity, layer);
-    k_bits_count<<<cb, kBlock, 0, s>>>(d, parity);
-    k_bits_scan<<<1, kBlock, 0, s>>>(d, parity, 1, blocks, grid_for(d.nnuclei, 1024));
-    k_bits_write<<<cb, kBlock, 0, s>>>(d, parity);
+    k_rebuild<<<cb, kBlock, 0, s>>>(d, parity, 1, blocks, grid_for(d.nnuclei, 1024));
 }
 void launch_rewake(const DomainView &d, int parity_next, int blocks, hipStream_t s) {
-    const int cb = static_cast<int>(d.nchunks < static_cast<uint32_t>(blocks) ? d.nchunks : blocks);
+    const int cb = rebuild_blocks(d, blocks);
     k_rewake<<<blocks, kBlock, 0, s>>>(d, parity_next);
-    k_bits_count<<<cb, kBlock, 0, s>>>(d, 1 - parity_next);
-    k_bits_scan<<<1, kBlock, 0, s>>>(d, 1 - parity_next, 0, blocks, 0);
-    k_bits_write<<<cb, kBlock, 0, s>>>(d, 1 - parity_next);
+    k_rebuild<<<cb, kBlock, 0, s>>>(d, 1 - parity_next, 0, blocks, 0);
 }
 void launch_step_timed(const DomainView &d, int parity, int layer, int blocks, hipStream_t s, hipEvent_t *ev) {
-    const int cb = static_cast<int>(d.nchunks < static_cast<uint32_t>(blocks) ? d.nchunks : blocks);
+    const int cb = rebuild_blocks(d, blocks);
     int e = 0;
     auto rec = [&]() { (void)hipEventRecord(ev[e++], s); };
     rec();
@@ -1356,13 +1403,7 @@ void launch_step_timed(const DomainView &d, int parity, int layer, int blocks, h
     k_settle<<<settle_blocks(blocks), kBlock, 0, s>>>(d, parity, layer);
     rec();
     rec();
-    k_bits_count<<<cb, kBlock, 0, s>>>(d, parity);
-    rec();
-    rec();
-    k_bits_scan<<<1, kBlock, 0, s>>>(d, parity, 1, blocks, grid_for(d.nnuclei, 1024));
-    rec();
-    rec();
-    k_bits_write<<<cb, kBlock, 0, s>>>(d, parity);
+    k_rebuild<<<cb, kBlock, 0, s>>>(d, parity, 1, blocks, grid_for(d.nnuclei, 1024));
     rec();
 }
 void launch_step_n(const DomainView *views, int slots, int parity, int layer, int nuc_blocks, int blocks,
@@ -1371,9 +1412,9 @@ void launch_step_n(const DomainView *views, int slots, int parity, int layer, in
     k_claim_n<<<dim3(blocks, slots), kBlock, 0, s>>>(views, parity, layer);
     k_apply_n<<<dim3(apply_blocks(blocks), slots), kBlock, 0, s>>>(views, parity, blocks);
     k_settle_n<<<dim3(settle_blocks(blocks), slots), kBlock, 0, s>>>(views, parity, layer);
-    k_bits_count_n<<<dim3(blocks, slots), kBlock, 0, s>>>(views, parity);
-    k_bits_scan_n<<<dim3(1, slots), kBlock, 0, s>>>(views, parity, blocks, nuc_blocks);
-    k_bits_write_n<<<dim3(blocks, slots), kBlock, 0, s>>>(views, parity);
+    // ensemble domains are small (a few dozen units at most: one round of 64 workgroups, each
+    // waiting only for workgroups dispatched before it) plus the booking workgroup
+    k_rebuild_n<<<dim3((blocks < 64 ? blocks : 64) + 1, slots), kBlock, 0, s>>>(views, parity, blocks, nuc_blocks);
 }
 void launch_check(const DomainView &d, int cycle, int layer, CheckResult *out, hipStream_t s) {
     k_check<<<grid_for(d.ncell, 4096), kBlock, 0, s>>>(d, cycle, layer, out);

@@ -26,8 +26,8 @@ void launch_check(const DomainView &d, int cycle, int layer, CheckResult *out, h
 // Octahedra of the initially active cells (src/CAinitialize.cpp:1936-2043).
 void launch_seed_init(const DomainView &d, const uint32_t *cells, uint32_t n, hipStream_t s);
 // One step with an event pair around each kernel: nucleation, claim, apply, settle, and the
-// frontier rebuild (count, scan, write). ev holds 2 * kTimedKernels events.
-constexpr int kTimedKernels = 7;
+// frontier rebuild. ev holds 2 * kTimedKernels events.
+constexpr int kTimedKernels = 5;
 void launch_step_timed(const DomainView &d, int parity, int layer, int blocks, hipStream_t s, hipEvent_t *ev);
 // Initial cell records from compact device arrays (CellType with the seed bit, LayerID, GrainID,
 // CritTimeStep), and one field back out of the records (FieldKind layout).

@@ -63,7 +63,8 @@ void upload_cells(const DomainView &d, const HostDomain &hd, hipStream_t s) {
     HIP_OK(hipMemcpyAsync(cts_d, hd.cts.data(), 4 * n, hipMemcpyHostToDevice, s));
     launch_pack(d, ct_d, lid_d, gid_d, cts_d, s);
     HIP_OK(hipGetLastError());
-    HIP_OK(hipMemsetAsync(d.alarms, 0, 4 * kAlarmSlots * ((n + 31) / 32), s));
+    HIP_OK(hipMemsetAsync(d.alarms, 0, 4 * kAlarmSlots * alarm_stride(d.ncell), s));
+    HIP_OK(hipMemsetAsync(d.chunk_flag, 0, 8ull * d.nchunks, s));
     if (!hd.active.empty()) {
         HIP_OK(hipMemcpyAsync(d.list[0], hd.active.data(), 4 * hd.active.size(), hipMemcpyHostToDevice, s));
         launch_seed_init(d, d.list[0], static_cast<uint32_t>(hd.active.size()), s);
@@ -174,8 +175,8 @@ void build_device_state(exaca_state *h) {
     d.slice_claims = h->alloc<uint32_t>(h->blocks);
     const size_t nwords = (n + 31) / 32;
     d.nchunks = static_cast<uint32_t>((nwords + 2047) / 2048);
-    d.alarms = h->alloc<uint32_t>(kAlarmSlots * nwords);
-    d.chunk_sum = h->alloc<uint32_t>(d.nchunks);
+    d.alarms = h->alloc<uint32_t>(kAlarmSlots * alarm_stride(d.ncell));
+    d.chunk_flag = h->alloc<unsigned long long>(d.nchunks);
     d.nnuclei = static_cast<int32_t>(hd.nloc.size());
     int32_t *nloc = h->alloc<int32_t>(hd.nloc.size());
     int32_t *ntime = h->alloc<int32_t>(hd.ntime.size());
@@ -537,7 +538,7 @@ int exaca_profile_steps(exaca_state *h, int layer, int nsteps, double ms[3], int
     });
 }
 
-int exaca_profile_kernels(exaca_state *h, int layer, int nsteps, double ms[7], int64_t counts[5]) {
+int exaca_profile_kernels(exaca_state *h, int layer, int nsteps, double ms[5], int64_t counts[5]) {
     return guarded([&] {
         if (!h || !ms || !counts || nsteps < 0) throw std::invalid_argument("bad argument");
         if (h->pending >= 0) throw StateError("profile_kernels called between nucleation and capture");

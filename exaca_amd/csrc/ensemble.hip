@@ -88,7 +88,7 @@ size_t domain_bytes(const exaca::HostDomain &h, size_t ngrow) {
     b += 2 * up(4 * std::max<size_t>(h.nloc.size(), 1)) + up(8 * ngrow);
     b += up(sizeof(exaca::StepCtl)) + up(sizeof(exaca::StepStats)) + up(8ull * exaca::kPartRows * kMaxSlices);
     const size_t nwords = (n + 31) / 32;
-    b += up(4 * exaca::kAlarmSlots * nwords) + up(4 * ((nwords + 2047) / 2048));  // calendar, chunk sums
+    b += up(4 * exaca::kAlarmSlots * exaca::alarm_stride(h.ncell)) + up(8 * ((nwords + 2047) / 2048));  // calendar, chunk flags
     return b;
 }
 
@@ -207,8 +207,8 @@ void admit(exaca_ensemble *e, const exaca_prepared *pr, int *slot_out) {
     double *grow = reinterpret_cast<double *>(carve(8 * pr->grow.size()));
     const size_t nwords = (n + 31) / 32;
     d.nchunks = static_cast<uint32_t>((nwords + 2047) / 2048);
-    d.alarms = reinterpret_cast<uint32_t *>(carve(4 * kAlarmSlots * nwords));
-    d.chunk_sum = reinterpret_cast<uint32_t *>(carve(4 * d.nchunks));
+    d.alarms = reinterpret_cast<uint32_t *>(carve(4 * kAlarmSlots * alarm_stride(d.ncell)));
+    d.chunk_flag = reinterpret_cast<unsigned long long *>(carve(8ull * d.nchunks));
     d.ctl = reinterpret_cast<StepCtl *>(carve(sizeof(StepCtl)));
     d.stats = reinterpret_cast<StepStats *>(carve(sizeof(StepStats)));
     d.nparts = kMaxSlices;
