@@ -50,6 +50,11 @@ constexpr uint16_t kCounted = 1;
 // Steps ahead a cell can be scheduled; a cell with nothing to do for longer is scheduled at the
 // horizon and checks again.
 constexpr uint32_t kAlarmSlots = 64;
+// The apply pass: one workgroup per kApplyGroup capture-pass workgroups, each with a region of
+// kCandStage solidification candidates.
+constexpr uint32_t kApplyGroup = 4;
+constexpr uint32_t kCandStage = 2048;
+constexpr uint32_t apply_blocks(uint32_t claim_blocks) { return (claim_blocks + kApplyGroup - 1) / kApplyGroup; }
 // Words of one calendar slot: one bit per cell, rounded up to 32-byte rows (the list rebuild reads
 // a slot in 32-byte pieces).
 constexpr size_t alarm_stride(long long ncell) {
@@ -186,6 +191,8 @@ struct DomainView {
     uint32_t *claim_cell;   // cells with claims in the current step, stored in their own frontier
     uint32_t *claim_bits;   //   slice, and the neighbours each claimed
     uint32_t *slice_claims; // entries of each workgroup's slice of the claim lists
+    uint32_t *cand;         // kCandStage solidification candidates per apply workgroup
+    uint32_t *cand_count;   // candidates in each region
     uint32_t list_cap;
     int32_t nnuclei;
     const int32_t *nloc;

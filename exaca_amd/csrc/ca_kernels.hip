@@ -636,12 +636,9 @@ __device__ __forceinline__ void claim_body(const DomainView &d, int p, int layer
 // leaves the liquid masks of its neighbours. Only masks that still hold it are updated: the masks
 // are read first, and a cell's bit is cleared by its capture alone, so the read decides exactly.
 // A neighbour whose mask this empties is a solidification candidate of this step, staged per
-// workgroup and appended to the candidate list held in the other list buffer.
-constexpr uint32_t kApplyGroup = 4;
-constexpr uint32_t kCandStage = 2048;  // candidates staged per workgroup
-__host__ __device__ constexpr uint32_t apply_blocks(uint32_t claim_blocks) {
-    return (claim_blocks + kApplyGroup - 1) / kApplyGroup;
-}
+// workgroup in LDS and written to the workgroup's own region of the candidate buffer (no shared
+// counter); a wave that finds the stage full appends to the overflow list in the other list buffer.
+
 
 __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t a, uint32_t nbc) {
     __shared__ unsigned long long sh[kBlock / 64];
@@ -655,6 +652,7 @@ __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t 
     StepCtl *ctl = d.ctl;
     if (ctl->done) return;
     if (threadIdx.x == 0) ncstage = 0;
+    uint32_t *overflow = d.list[1 - p];
     uint32_t beg[kApplyGroup], upto[kApplyGroup], n = 0;
 #pragma unroll
     for (uint32_t g = 0; g < kApplyGroup; g++) {
@@ -670,7 +668,6 @@ __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t 
     }
     const uint32_t serial = ctl->serial[p];
     const Grid gr = grid_of(d);
-    uint32_t *cand = d.list[1 - p];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     unsigned long long ncap = 0;
     uint32_t qlo = 0xFFFFFFFFu, qhi = 0;
@@ -801,19 +798,16 @@ __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t 
                     if (staged)
                         cstage[pos] = cell;
                     else if (pos < d.list_cap)
-                        cand[pos] = cell;
+                        overflow[pos] = cell;
                 }
             }
         }
         __syncthreads();
     }
-    // The staged candidates go to the global list with one atomic per workgroup.
-    __shared__ uint32_t cglobal;
+    // The staged candidates go to this workgroup's region.
     const uint32_t staged = ncstage < kCandStage ? ncstage : kCandStage;
-    if (threadIdx.x == 0) cglobal = staged ? atomicAdd(&ctl->ncand, staged) : 0;
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < staged; i += kBlock)
-        if (cglobal + i < d.list_cap) cand[cglobal + i] = cstage[i];
+    for (uint32_t i = threadIdx.x; i < staged; i += kBlock) d.cand[a * kCandStage + i] = cstage[i];
+    if (threadIdx.x == 0) d.cand_count[a] = staged;
     put_part(d, kPartCap, a, ncap, sh);
     // New cells widen the range of active cells (least and greatest index of this workgroup).
     __shared__ uint32_t lohi[2 * (kBlock / 64)];
@@ -841,10 +835,9 @@ __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t 
     }
 }
 
-// The settle pass has a few percent of the capture pass's cells: a quarter of its workgroups.
-__host__ __device__ constexpr uint32_t settle_blocks(uint32_t claim_blocks) {
-    return claim_blocks / 4 > 8 ? claim_blocks / 4 : (claim_blocks < 8 ? claim_blocks : 8);
-}
+// The settle pass has one workgroup per apply workgroup: workgroup b takes the candidates of
+// region b and a share of the overflow list.
+__host__ __device__ constexpr uint32_t settle_blocks(uint32_t claim_blocks) { return apply_blocks(claim_blocks); }
 
 // Pass C over the candidates: a cell whose every claim-time liquid neighbour was captured in this
 // step by a lower-indexed cell has no liquid neighbour left when the reference visits it in index
@@ -859,26 +852,27 @@ __device__ __forceinline__ void settle_body(const DomainView &d, int p, int laye
     if (ctl->done) return;
     const int cycle = ctl->cycle[p];
     const uint32_t serial = ctl->serial[p];
-    const uint32_t n = ctl->ncand < d.list_cap ? ctl->ncand : d.list_cap;
-    const uint32_t *cand = d.list[1 - p];
-    StepCounts c{0, 0, 0, 0};
+    const uint32_t nown = d.cand_count[b];
+    const uint32_t nover = ctl->ncand < d.list_cap ? ctl->ncand : d.list_cap;
+    const uint32_t *own = d.cand + b * kCandStage;
+    const uint32_t *overflow = d.list[1 - p];
+    unsigned long long nsol = 0;
     __shared__ double gtab[kGrowStage];
     const Growth g = staged_growth(d, gtab);
-    for (uint32_t i = b * kBlock + threadIdx.x; i < n; i += nb * kBlock) {
-        const uint32_t r = cand[i];
+    auto settle = [&](uint32_t r) {
         const CellWords w = load_cell(d.cells + r);
-        if ((w.ct & kTypeMask) != kActive) continue;
-        if (!(cycle > w.cts && w.lid <= layer)) continue;
-        const bool keep = static_cast<uint32_t>(w.lm >> 32) == serial + 1;
-        if (!keep) {
-            solidify(d, g, r, w, growth_index(d, w.cts, cycle));
-            c.nsol++;
-        }
-        else {
+        if ((w.ct & kTypeMask) != kActive || !(cycle > w.cts && w.lid <= layer)) return;
+        if (static_cast<uint32_t>(w.lm >> 32) == serial + 1) {
             set_alarm(d, serial + 1, r);
         }
-    }
-    put_part(d, kPartSettleSol, b, c.nsol, sh);
+        else {
+            solidify(d, g, r, w, growth_index(d, w.cts, cycle));
+            nsol++;
+        }
+    };
+    for (uint32_t i = threadIdx.x; i < nown; i += kBlock) settle(own[i]);
+    for (uint32_t i = b * kBlock + threadIdx.x; i < nover; i += nb * kBlock) settle(overflow[i]);
+    put_part(d, kPartSettleSol, b, nsol, sh);
 }
 
 

@@ -173,6 +173,8 @@ void build_device_state(exaca_state *h) {
     d.claim_cell = h->alloc<uint32_t>(n);
     d.claim_bits = h->alloc<uint32_t>(n);
     d.slice_claims = h->alloc<uint32_t>(h->blocks);
+    d.cand = h->alloc<uint32_t>(static_cast<size_t>(apply_blocks(h->blocks)) * kCandStage);
+    d.cand_count = h->alloc<uint32_t>(apply_blocks(h->blocks));
     const size_t nwords = (n + 31) / 32;
     d.nchunks = static_cast<uint32_t>((nwords + 2047) / 2048);
     d.alarms = h->alloc<uint32_t>(kAlarmSlots * alarm_stride(d.ncell));

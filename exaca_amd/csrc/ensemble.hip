@@ -85,6 +85,7 @@ size_t domain_bytes(const exaca::HostDomain &h, size_t ngrow) {
     size_t b = 0;
     b += up(sizeof(exaca::Cell) * n) + up(sizeof(exaca::Centre) * n);  // cell records
     b += 4 * up(4 * n) + up(4 * kMaxSlices);                            // frontier, claim lists
+    b += up(4ull * exaca::apply_blocks(kMaxSlices) * exaca::kCandStage) + up(4 * exaca::apply_blocks(kMaxSlices));
     b += 2 * up(4 * std::max<size_t>(h.nloc.size(), 1)) + up(8 * ngrow);
     b += up(sizeof(exaca::StepCtl)) + up(sizeof(exaca::StepStats)) + up(8ull * exaca::kPartRows * kMaxSlices);
     const size_t nwords = (n + 31) / 32;
@@ -201,6 +202,8 @@ void admit(exaca_ensemble *e, const exaca_prepared *pr, int *slot_out) {
     d.claim_cell = reinterpret_cast<uint32_t *>(carve(4 * n));
     d.claim_bits = reinterpret_cast<uint32_t *>(carve(4 * n));
     d.slice_claims = reinterpret_cast<uint32_t *>(carve(4 * kMaxSlices));
+    d.cand = reinterpret_cast<uint32_t *>(carve(4ull * exaca::apply_blocks(kMaxSlices) * exaca::kCandStage));
+    d.cand_count = reinterpret_cast<uint32_t *>(carve(4 * exaca::apply_blocks(kMaxSlices)));
     d.list_cap = static_cast<uint32_t>(n);
     int32_t *nloc = reinterpret_cast<int32_t *>(carve(4 * std::max<size_t>(hd.nloc.size(), 1)));
     int32_t *ntime = reinterpret_cast<int32_t *>(carve(4 * std::max<size_t>(hd.nloc.size(), 1)));
