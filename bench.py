@@ -8,10 +8,13 @@ exactly --steps steps run from the device-resident state (HIP graphs, terminatio
 timed steps of the frontier size (cells past their liquidus time, the reference's numSteer)
 divided by the device time of those steps.
 
-roofline: the dominant work is CellCapture (k_claim, k_apply, k_settle and the frontier rebuild).
-Its algorithmic bytes are 137 B per active-cell update + 125 B per capture + 1 B per
-solidification (the reference layout's bytes: own state, the 26 critical lengths and the new cell's
-octahedron), divided by the kernels' event-timed durations over --profile-steps further steps.
+roofline: the dominant work is CellCapture (k_claim, k_apply, k_settle and k_rebuild, the
+reference's CellCapture kernel). Its algorithmic bytes are 137 B per active-cell update + 125 B per
+capture + 1 B per solidification (the reference layout's bytes: own state, the 26 critical lengths
+and the new cell's octahedron), divided by the kernels' event-timed durations over --profile-steps
+further steps. traffic is the measured HBM bytes per step of the same kernels (rocprofv3 FETCH_SIZE
+and WRITE_SIZE, calibrated on a plain copy) from the committed summary
+profiles/capture_kernels_800_hbm_traffic.md.
 
 cpu_baseline: the oracle (the serial C restatement of the reference's algorithm) on a bounded
 sample of the same workload: the first --cpu-cycles steps of the same deck cut to its bottom
@@ -39,6 +42,17 @@ sys.path.insert(0, str(REPO))
 
 METRIC = "active-cell updates/s + HBM GB/s at 800³; ensemble solves/s, p99 vs plain loop"
 HBM_PEAK_GBPS = 8000.0
+TRAFFIC_PROFILE = REPO / "profiles" / "capture_kernels_800_hbm_traffic.md"
+
+
+def measured_traffic() -> int | None:
+    """HBM bytes per step of the CellCapture kernels from the committed rocprofv3 summary."""
+    if not TRAFFIC_PROFILE.exists():
+        return None
+    for ln in TRAFFIC_PROFILE.read_text().splitlines():
+        if ln.startswith("traffic_bytes_per_step:"):
+            return int(ln.split(":")[1])
+    return None
 
 
 def cpu_baseline(deck: Path, z: int, cycles: int) -> dict:
@@ -145,14 +159,13 @@ def main() -> int:
     prof = sol.profile_kernels(args.profile_steps)
     kms = prof["ms"]
     nsteps = max(prof["steps"], 1)
-    # CellCapture = claim + apply + the rebuild of the frontier (the reference's pass 1 and pass 2)
+    # CellCapture = claim + apply + settle + the rebuild of the next list (the reference's pass 1
+    # and pass 2)
     cap_ms = sum(v for k, v in kms.items() if k != "k_nucleate")
     bytes_alg = 137 * prof["frontier"] + 125 * prof["captures"] + 1 * prof["solidified"]
     achieved = bytes_alg / (cap_ms / 1e3) / 1e9 if cap_ms > 0 else 0.0
-    # k_claim alone with the bytes this layout moves per gated frontier cell: list 4,
-    # CritTimeStep 4, LayerID 1, DiagonalLength 4 + 4, capture threshold 4, liquid mask 8
-    claim_bytes = 29 * prof["frontier"]
-    claim_gbps = claim_bytes / (kms["k_claim"] / 1e3) / 1e9 if kms["k_claim"] > 0 else 0.0
+    traffic = measured_traffic()
+    step_us = 1e3 * cap_ms / nsteps
     sol.close()
 
     line = {
@@ -179,25 +192,21 @@ def main() -> int:
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "CellCapture = k_claim + k_apply + k_settle + frontier rebuild (k_bits_count, k_bits_scan, "
-                      "k_bits_write)",
+            "kernel": "CellCapture = k_claim + k_apply + k_settle + k_rebuild (one step)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
-            "traffic": None,
-            "avg_launch_us": 1e3 * cap_ms / nsteps,
+            "traffic": traffic,
+            "traffic_source": str(TRAFFIC_PROFILE.relative_to(REPO)) if traffic else None,
+            "avg_launch_us": step_us,
             "profiled_steps": prof["steps"],
             "bytes_per_update": 137,
             "bytes_per_capture": 125,
+            "bytes_alg_per_step": bytes_alg / nsteps,
         },
-        "roofline_k_claim_own_layout": {
-            "achieved": claim_gbps,
-            "peak": HBM_PEAK_GBPS,
-            "unit": "GB/s",
-            "frac": claim_gbps / HBM_PEAK_GBPS,
-            "bytes_per_update": 29,
-        },
+        # the measured HBM bytes of a step over the same event-timed duration
+        "measured_hbm_GBps": traffic / (step_us * 1e3) if traffic and step_us > 0 else None,
         "kernel_avg_us": {k: 1e3 * v / nsteps for k, v in kms.items()},
         "per_step": {k: (q1[k] - q0[k]) / max(steps, 1)
                      for k in ("frontier", "captures", "solidified", "evaluated", "masks")},
