@@ -4,21 +4,22 @@
 // reference linear order p = z*X*Y + x*Y + y with y fastest (src/CAupdate.cpp:220-225). Active
 // cells are sparse in that order (a front of a few percent of each plane over a band of planes),
 // so everything a step reads or atomically updates for a cell sits in one aligned 32-byte sector:
-//   lm   uint64  liquid neighbours of an active cell. Bits 0-25: now; bit 31 set: not yet known
-//                (a new cell, or never active). Bits 32-63: serial + 1 of the last step in which
-//                a cell of index >= this one captured one of its liquid neighbours (keep tag).
+//   lm   uint64  active cell: liquid neighbours in bits 0-25 (bit 31 set: not yet known, for a
+//                nucleus or an initial seed); bits 32-63: serial + 1 of the last step in which a
+//                cell of index >= this one captured one of its liquid neighbours (keep tag).
+//                Liquid cell: low word = liquid marker (bits 26-31 set), high word = claim word,
+//                the lowest index of a cell that claimed it in the current step (0xFFFFFFFF:
+//                none). A claimed cell is captured in the same step and never turns liquid again,
+//                so a claim word is read only in the step that wrote it.
 //   dl   float   octahedron half-diagonal (DiagonalLength) after the growth of steps up to `win`
 //   thr  float   lower bound on the critical lengths of the still-liquid neighbours (0: unknown)
 //   cts  int32   CritTimeStep (liquidus crossing step)
 //   gid  int32   GrainID
-//   win  uint32  liquid cell: lowest index of a cell that claimed it in the current step
-//                (0xFFFFFFFF: unclaimed). A claimed cell is captured in the same step and never
-//                turns liquid again, so a claim word is read in the step that wrote it only; from
-//                its first step as an active cell on, the word holds the growth index: dl includes
-//                the growth of every executed step up to that many steps past the liquidus time.
+//   win  uint32  active cell: the growth index: dl includes the growth of every executed step up
+//                to that many steps past the liquidus time (not yet set while kFresh or unknown)
 //   lid  int8    LayerID
 //   ct   uint8   cell type (values of src/CAtypes.hpp:11) | kSeedBit for cell-centred octahedra
-//   flags        kCounted
+//   flags        kCounted, kFresh
 // The octahedron centre (double, as computed at capture; DOCenter is its float rounding) is a
 // second 32-byte record per cell, read only when a cell evaluates critical lengths or captures.
 // The reference's 26 critical diagonal lengths per cell (104 B) are not stored: they are
@@ -47,6 +48,7 @@ constexpr int kNumOrientations = 10000;
 constexpr unsigned long long kLmUnknown = 1ull << 31;
 constexpr unsigned long long kLmNow = (1ull << 26) - 1;
 constexpr uint16_t kCounted = 1;
+constexpr uint16_t kFresh = 2;  // mask built by the cell's capture; not yet looked at as an active cell
 // Steps ahead a cell can be scheduled; a cell with nothing to do for longer is scheduled at the
 // horizon and checks again.
 constexpr uint32_t kAlarmSlots = 64;
@@ -143,7 +145,7 @@ struct alignas(32) Cell {
     uint32_t win;
     int8_t lid;
     uint8_t ct;
-    uint16_t flags;  // kCounted: past its liquidus time and counted in StepCtl::nonwaiting
+    uint16_t flags;  // kCounted: past its liquidus time and counted in StepCtl::nonwaiting; kFresh
 };
 static_assert(sizeof(Cell) == 32, "one sector per cell");
 struct alignas(32) Centre {

@@ -323,6 +323,16 @@ __device__ __forceinline__ uint32_t *keep_word(const DomainView &d, uint32_t q) 
 }
 constexpr uint32_t kMaskUnknown = static_cast<uint32_t>(kLmUnknown);
 constexpr uint32_t kMaskNow = static_cast<uint32_t>(kLmNow);
+// A liquid cell's words: the low word keeps bits 26-31 of its initial all-ones value (only bits
+// 0-25 are ever cleared, by nucleation), bit 30 marking it liquid; the high word is its claim word:
+// the lowest index of a cell that claimed it in the current step (kClaimNone: none). A claimed
+// cell is captured in the same step and never turns liquid again, so a claim word is read only in
+// the step that wrote it, and the capture overwrites both words.
+constexpr uint32_t kLiquidMark = 1u << 30;
+__device__ __forceinline__ uint32_t *claim_word(const DomainView &d, uint32_t q) { return keep_word(d, q); }
+__device__ __forceinline__ bool is_unclaimed_liquid(unsigned long long lm) {
+    return (static_cast<uint32_t>(lm) & kLiquidMark) && static_cast<uint32_t>(lm >> 32) == kClaimNone;
+}
 
 // ---- Growth between events. A cell's half-diagonal grows by grow[k] at the k-th executed step
 // past its liquidus time (src/CAupdate.cpp:257-261). Between two events of a cell the growth is
@@ -396,12 +406,21 @@ __device__ __forceinline__ void put_part(const DomainView &d, int row, uint32_t 
     if (threadIdx.x == 0) d.part[static_cast<size_t>(row) * d.nparts + b] = t;
 }
 
+// Growth index that an active cell's dl includes, at a step of growth index k. A cell not yet
+// looked at as an active cell (mask not yet known, or built by its capture: kFresh) holds its
+// initial dl: it grows from the step it is first looked at on, so its base is k - 1 (0 while it
+// waits for its liquidus time); otherwise the base is in win.
+__device__ __forceinline__ int growth_base(const CellWords &w, int k, bool waiting) {
+    if ((w.lm & kLmUnknown) || (w.flags & kFresh)) return waiting ? 0 : k - 1;
+    return static_cast<int>(w.win);
+}
 // Cell r of known mask is past its liquidus time and has no liquid neighbour left: at its visit
 // in this step's capture the reference grows it and turns it solid (src/CAupdate.cpp:722-725).
 __device__ __forceinline__ void solidify(const DomainView &d, const Growth &g, uint32_t r, const CellWords &w, int k) {
     Cell &c = d.cells[r];
-    c.dl = replay_growth(g, w.dl, static_cast<int>(w.win), k);
+    c.dl = replay_growth(g, w.dl, growth_base(w, k, false), k);
     c.win = static_cast<uint32_t>(k);
+    c.flags = w.flags & ~kFresh;
     c.ct = kSolid;
 }
 
@@ -454,8 +473,8 @@ __device__ __forceinline__ void nucleate_body(const DomainView &d, int p, int la
             const CellWords w = load_cell(d.cells + r);
             if ((w.ct & kTypeMask) != kActive || !(cycle > w.cts && w.lid <= layer)) continue;
             if (!(w.flags & kCounted)) n.entered++;
-            d.cells[r].flags = w.flags | kCounted;
             solidify(d, growth_of(d), r, w, growth_index(d, w.cts, cycle));
+            d.cells[r].flags = (w.flags & ~kFresh) | kCounted;
             n.nsol++;
         }
         atomicMin(&ctl->elo, q);
@@ -491,7 +510,7 @@ __device__ uint32_t evaluate_claims(const DomainView &d, const Grid &gr, uint32_
             const int l = __builtin_ctz(m);
             const float cdl = tab[l];
             if (dlj >= cdl) {
-                atomicMin(&d.cells[j + gr.offset(l)].win, j);
+                atomicMin(claim_word(d, j + gr.offset(l)), j);
                 claimed |= 1u << l;
             }
             else {
@@ -508,7 +527,7 @@ __device__ uint32_t evaluate_claims(const DomainView &d, const Grid &gr, uint32_
             const int l = __builtin_ctz(m);
             const float cdl = critical_length(f, xp, yp, zp, e.x, e.y, e.z, l);
             if (dlj >= cdl) {
-                atomicMin(&d.cells[j + gr.offset(l)].win, j);
+                atomicMin(claim_word(d, j + gr.offset(l)), j);
                 claimed |= 1u << l;
             }
             else {
@@ -561,7 +580,7 @@ __device__ __forceinline__ void claim_body(const DomainView &d, int p, int layer
                 const bool waiting = !(cycle > w.cts && w.lid <= layer);
                 const int k = growth_index(d, w.cts, cycle);
                 uint32_t liquid;
-                int kdl;
+                const int kdl = growth_base(w, k, waiting);
                 if (w.lm & kLmUnknown) {
                     const Coord c = decode(j, gr);
                     const uint32_t valid = valid_neighbours(c, gr.X, gr.Y, gr.nzA);
@@ -573,15 +592,14 @@ __device__ __forceinline__ void claim_body(const DomainView &d, int p, int layer
                     }
                     d.cells[j].lm = liquid;
                     n.nmask++;
-                    kdl = waiting ? 0 : k - 1;
                 }
                 else {
                     liquid = static_cast<uint32_t>(w.lm & kLmNow);
-                    kdl = static_cast<int>(w.win);
                 }
                 Cell &cj = d.cells[j];
                 if (waiting) {
                     cj.win = static_cast<uint32_t>(kdl);
+                    if (w.flags & kFresh) cj.flags = w.flags & ~kFresh;
                     const int ahead = w.cts + 1 - cycle;
                     const uint32_t wake = ahead < 1 ? 1u
                                                     : (ahead < static_cast<int>(kAlarmSlots) - 1 ? static_cast<uint32_t>(ahead)
@@ -589,10 +607,8 @@ __device__ __forceinline__ void claim_body(const DomainView &d, int p, int layer
                     set_alarm(d, serial + wake, j);
                 }
                 else {
-                    if (!(w.flags & kCounted)) {
-                        n.entered++;
-                        cj.flags = w.flags | kCounted;
-                    }
+                    if (!(w.flags & kCounted)) n.entered++;
+                    if ((w.flags & (kCounted | kFresh)) != kCounted) cj.flags = (w.flags & ~kFresh) | kCounted;
                     const float dlj = replay_growth(g, w.dl, kdl, k);
                     cj.dl = dlj;
                     cj.win = static_cast<uint32_t>(k);
@@ -690,7 +706,7 @@ __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t 
             won_cz[threadIdx.x] = static_cast<float>(e.z);
             for (uint32_t m = d.claim_bits[i]; m; m &= m - 1) {
                 const int l = __builtin_ctz(m);
-                won |= static_cast<uint32_t>(d.cells[j + gr.offset(l)].win == j) << l;
+                won |= static_cast<uint32_t>(*claim_word(d, j + gr.offset(l)) == j) << l;
             }
         }
         const uint32_t cnt = __builtin_popcount(won);
@@ -731,9 +747,13 @@ __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t 
                 const uint32_t valid = valid_neighbours(cqc, gr.X, gr.Y, gr.nzA);
                 // the neighbours' masks are loaded before the octahedron is built (they arrive
                 // while it is computed)
-                uint32_t mw[26];
+                // both words: the low word is the mask (or liquid marker), the high word of a liquid
+                // neighbour its claim word
+                unsigned long long lw[26];
 #pragma unroll
-                for (int kk = 0; kk < 26; kk++) mw[kk] = ((valid >> kk) & 1u) ? *mask_word(d, q + gr.off[kk]) : kMaskUnknown;
+                for (int kk = 0; kk < 26; kk++)
+                    lw[kk] = ((valid >> kk) & 1u) ? *reinterpret_cast<const unsigned long long *>(&d.cells[q + gr.off[kk]].lm)
+                                                  : static_cast<unsigned long long>(kMaskUnknown);
                 const int h = won_gid[lo];
                 int o = orientation_of(h);
                 if (o < 0) {
@@ -748,14 +768,23 @@ __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t 
                 cq.gid = h;
                 cq.dl = static_cast<float>(oc.dl);
                 cq.thr = 0.0f;
-                cq.lm = kLmUnknown;
-                cq.flags = 0;
+                cq.flags = kFresh;
                 cq.ct = kActive;
                 d.centre[q] = Centre{oc.cx, oc.cy, oc.cz, 0.0};
                 set_alarm(d, serial + 1, q);
                 ncap++;
                 qlo = q < qlo ? q : qlo;
                 qhi = q > qhi ? q : qhi;
+                // The new cell's mask: its liquid neighbours at its first visit are those liquid now
+                // and not claimed in this step (every claimed cell is captured in this step, and the
+                // claims are final). Until its store below, the new cell's own words read as a
+                // claimed liquid cell, which no other capture counts.
+                uint32_t liquid = 0, mw[26];
+#pragma unroll
+                for (int kk = 0; kk < 26; kk++) {
+                    liquid |= static_cast<uint32_t>(is_unclaimed_liquid(lw[kk])) << kk;
+                    mw[kk] = static_cast<uint32_t>(lw[kk]);
+                }
                 // updates only where the mask still holds q, all in flight together, then the results
                 uint32_t need = 0;
 #pragma unroll
@@ -777,6 +806,7 @@ __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t 
                     const uint32_t bit = 1u << kOpposite.v[kk];
                     emptied |= static_cast<uint32_t>(((need >> kk) & 1u) && (mw[kk] & (kMaskUnknown | kMaskNow)) == bit) << kk;
                 }
+                cq.lm = liquid;
             }
             // Candidates: staged in LDS (one LDS atomic per wave); a wave that finds the stage
             // full appends to the global list directly.
@@ -1287,8 +1317,10 @@ __global__ __launch_bounds__(kBlock) void k_pack(DomainView d, const uint8_t *ct
     for (long long q = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x; q < d.ncell;
          q += static_cast<long long>(gridDim.x) * kBlock) {
         uint4 *c = reinterpret_cast<uint4 *>(d.cells + q);
-        c[0] = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u);  // mask not yet known; no octahedron
-        c[1] = make_uint4(static_cast<uint32_t>(cts[q]), static_cast<uint32_t>(gid[q]), kClaimNone,
+        const bool liquid = (ct[q] & kTypeMask) == kLiquid;
+        // a liquid cell: the liquid marker and no claim; any other cell: mask not yet known
+        c[0] = make_uint4(liquid ? 0xFFFFFFFFu : kMaskUnknown, liquid ? kClaimNone : 0u, 0u, 0u);
+        c[1] = make_uint4(static_cast<uint32_t>(cts[q]), static_cast<uint32_t>(gid[q]), 0u,
                           static_cast<uint32_t>(static_cast<uint8_t>(lid[q])) | (static_cast<uint32_t>(ct[q]) << 8));
         d.centre[q] = Centre{0.0, 0.0, 0.0, 0.0};
     }
@@ -1305,7 +1337,7 @@ __global__ __launch_bounds__(kBlock) void k_extract(DomainView d, int field, voi
         case kFieldGrainId: static_cast<int32_t *>(out)[q] = c.gid; break;
         case kFieldDiagonal: {  // float, with the growth of the steps since the cell was last looked at
             float dl = c.dl;
-            if ((c.ct & kTypeMask) == kActive && !(c.lm & kLmUnknown))
+            if ((c.ct & kTypeMask) == kActive && !(c.lm & kLmUnknown) && !(c.flags & kFresh))
                 dl = replay_growth(growth_of(d), dl, static_cast<int>(c.win), growth_index(d, c.cts, d.ctl->last_cycle));
             static_cast<float *>(out)[q] = dl;
             break;
