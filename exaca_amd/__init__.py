@@ -27,6 +27,7 @@ FIELD_LAYERID = 5
 FIELD_CELLTYPE_RAW = 6
 FIELD_NUCLEI_LOC = 7
 FIELD_NUCLEI_TIME = 8
+FIELD_MELTED = 9
 
 # exported entry points and their signatures (include/exaca_ca.h)
 _I64P = ctypes.POINTER(ctypes.c_int64)
@@ -68,6 +69,8 @@ _SIGNATURES = {
     "exaca_ensemble_download": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                                ctypes.c_size_t]),
     "exaca_ensemble_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "exaca_grain_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32), _I64P,
+                                         ctypes.POINTER(ctypes.c_double), _I64P]),
 }
 
 _lib = None
@@ -288,6 +291,22 @@ class Solver:
             out = np.empty(self.ncell, dtype=np.int32)
         _ok(self._lib.exaca_download(self._h, kind, out.ctypes.data, out.nbytes))
         return out
+
+    def grain_stats(self, region=None) -> dict:
+        """Grain statistics of a region (XMin, XMax, YMin, YMax, ZMin, ZMax, inclusive, in the
+        coordinates of the reference's vtk files; default: every cell inside the walls), computed on
+        the GPU from the current GrainID field (exaca_grain_stats)."""
+        q = self.query()
+        if region is None:
+            region = (1, q["nx"] - 2, 1, q["ny"] - 2, 1, q["nz"] - 2)
+        reg = (ctypes.c_int32 * 6)(*region)
+        counts = (ctypes.c_int64 * 5)()
+        values = (ctypes.c_double * 4)()
+        hist = (ctypes.c_int64 * 64)()
+        _ok(self._lib.exaca_grain_stats(self._h, reg, counts, values, hist))
+        return {"cells": counts[0], "melted": counts[1], "grains": counts[2], "nucleated_cells": counts[3],
+                "melted_top": counts[4], "mean_misorientation": values[0], "mean_misorientation_top": values[1],
+                "mean_grain_volume": values[2], "nucleated_fraction": values[3], "histogram": list(hist)}
 
     def query(self) -> dict:
         out = (ctypes.c_int64 * 18)()

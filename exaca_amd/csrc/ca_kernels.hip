@@ -704,10 +704,13 @@ __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t 
             won_cx[threadIdx.x] = static_cast<float>(e.x);
             won_cy[threadIdx.x] = static_cast<float>(e.y);
             won_cz[threadIdx.x] = static_cast<float>(e.z);
-            for (uint32_t m = d.claim_bits[i]; m; m &= m - 1) {
-                const int l = __builtin_ctz(m);
-                won |= static_cast<uint32_t>(*claim_word(d, j + gr.offset(l)) == j) << l;
-            }
+            // the claim words of every claimed neighbour, loaded together
+            const uint32_t cb = d.claim_bits[i];
+            uint32_t cw[26];
+#pragma unroll
+            for (int l = 0; l < 26; l++) cw[l] = ((cb >> l) & 1u) ? *claim_word(d, j + gr.off[l]) : kClaimNone;
+#pragma unroll
+            for (int l = 0; l < 26; l++) won |= static_cast<uint32_t>(cw[l] == j) << l;
         }
         const uint32_t cnt = __builtin_popcount(won);
         uint32_t wtot;
@@ -1310,6 +1313,61 @@ __global__ __launch_bounds__(kBlock) void k_seed_init(DomainView d, const uint32
     }
 }
 
+// Grain statistics of a region of the whole-domain views (k, i, j), the layout the reference's vtk
+// files and its analysis executable use (analysis/src/GAprint.cpp:17-108): the interior of this
+// domain for z = 1 .. nz-2, zero on the bottom and top planes. Per cell: its GrainID is flagged in
+// gflag (GrainID - gmin; plain stores: every writer writes 1), a negative one counts as nucleated,
+// and a melted cell counts in the bin of its orientation (LDS bins flushed once per workgroup), in
+// a second set of bins when it lies in the region's top plane.
+__global__ __launch_bounds__(kBlock) void k_grain_stats(DomainView d, const uint8_t *melted, GrainRegion r, int gmin,
+                                                        uint32_t ngid, uint32_t *gflag, unsigned long long *counts,
+                                                        unsigned int *obins, unsigned int *otop) {
+    __shared__ unsigned int bins[kNumOrientations];
+    __shared__ unsigned long long sh[kBlock / 64];
+    for (int o = threadIdx.x; o < kNumOrientations; o += kBlock) bins[o] = 0;
+    __syncthreads();
+    const long long ni = r.i1 - r.i0 + 1, nj = r.j1 - r.j0 + 1, nk = r.k1 - r.k0 + 1;
+    const long long total = ni * nj * nk;
+    unsigned long long nmelt = 0, nnuc = 0, nbad = 0;
+    for (long long c = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x; c < total;
+         c += static_cast<long long>(gridDim.x) * kBlock) {
+        const int j = r.j0 + static_cast<int>(c % nj);
+        const int i = r.i0 + static_cast<int>((c / nj) % ni);
+        const int k = r.k0 + static_cast<int>(c / (nj * ni));
+        int g = 0;
+        bool m = false;
+        if (k >= 1 && k <= d.nz - 2) {
+            const long long q = (static_cast<long long>(k) * d.X + (i + 1)) * d.Y + (j + 1);
+            g = d.cells[q].gid;
+            m = melted[q] != 0;
+        }
+        const long long f = static_cast<long long>(g) - gmin;
+        if (f < 0 || f >= ngid) {
+            nbad++;
+            continue;
+        }
+        gflag[f] = 1u;
+        nnuc += g < 0;
+        if (m) {
+            nmelt++;
+            const int o = orientation_of(g) < 0 ? 0 : orientation_of(g);
+            atomicAdd(&bins[o], 1u);
+            if (k == r.k1) atomicAdd(&otop[o], 1u);
+        }
+    }
+    const unsigned long long tm = block_sum(nmelt, sh);
+    const unsigned long long tn = block_sum(nnuc, sh);
+    const unsigned long long tb = block_sum(nbad, sh);
+    if (threadIdx.x == 0) {
+        if (tm) atomicAdd(&counts[0], tm);
+        if (tn) atomicAdd(&counts[1], tn);
+        if (tb) atomicAdd(&counts[2], tb);
+    }
+    __syncthreads();
+    for (int o = threadIdx.x; o < kNumOrientations; o += kBlock)
+        if (bins[o]) atomicAdd(&obins[o], bins[o]);
+}
+
 // Initial cell records from the host's compact arrays: no octahedron, no claim, liquid mask not
 // yet known; the centres start at zero.
 __global__ __launch_bounds__(kBlock) void k_pack(DomainView d, const uint8_t *ct, const int8_t *lid,
@@ -1451,6 +1509,12 @@ void launch_seed_init(const DomainView &d, const uint32_t *cells, uint32_t n, hi
 void launch_pack(const DomainView &d, const uint8_t *ct, const int8_t *lid, const int32_t *gid, const int32_t *cts,
                  hipStream_t s) {
     k_pack<<<grid_for(d.ncell, 1 << 16), kBlock, 0, s>>>(d, ct, lid, gid, cts);
+}
+void launch_grain_stats(const DomainView &d, const uint8_t *melted, const GrainRegion &r, int gmin, uint32_t ngid,
+                        uint32_t *gflag, unsigned long long *counts, unsigned int *obins, unsigned int *otop,
+                        hipStream_t s) {
+    const long long total = static_cast<long long>(r.i1 - r.i0 + 1) * (r.j1 - r.j0 + 1) * (r.k1 - r.k0 + 1);
+    k_grain_stats<<<grid_for(total, 2048), kBlock, 0, s>>>(d, melted, r, gmin, ngid, gflag, counts, obins, otop);
 }
 void launch_extract(const DomainView &d, int field, void *out, hipStream_t s) {
     k_extract<<<grid_for(d.ncell, 1 << 16), kBlock, 0, s>>>(d, field, out);

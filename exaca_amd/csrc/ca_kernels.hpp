@@ -47,6 +47,16 @@ inline size_t field_bytes(int field) {
     default: return 0;
     }
 }
+// Grain statistics of the whole-domain region [k0, k1] x [i0, i1] x [j0, j1] (inclusive; the
+// reference's vtk / analysis layout): flags gflag[GrainID - gmin] (ngid entries), counts[0..2] +=
+// melted cells, cells with a negative GrainID, cells whose GrainID falls outside the flag range;
+// obins / otop += melted cells per orientation, in the region / in its top plane k1.
+struct GrainRegion {
+    int k0, k1, i0, i1, j0, j1;
+};
+void launch_grain_stats(const DomainView &d, const uint8_t *melted, const GrainRegion &r, int gmin, uint32_t ngid,
+                        uint32_t *gflag, unsigned long long *counts, unsigned int *obins, unsigned int *otop,
+                        hipStream_t s);
 // Plain copy of n 16-byte words (HBM calibration).
 void launch_copy(const uint4 *src, uint4 *dst, size_t n, hipStream_t s);
 // Device-side self test of the capture geometry against host values: one octahedron per input row.

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <chrono>
 #include <cstdlib>
 #include <memory>
@@ -492,6 +493,93 @@ int exaca_download(exaca_state *h, int field, void *dst, size_t dst_bytes) {
     });
 }
 
+int exaca_grain_stats(exaca_state *h, const int32_t region[6], int64_t counts[5], double values[4], int64_t hist[64]) {
+    return guarded([&] {
+        if (!h || !region) throw std::invalid_argument("null argument");
+        if (h->pending >= 0) throw StateError("grain_stats called between nucleation and capture");
+        const exaca::HostDomain &hd = h->host;
+        const int x0 = region[0], x1 = region[1], y0 = region[2], y1 = region[3], z0 = region[4], z1 = region[5];
+        if (x0 < 0 || x1 >= hd.nx || x0 > x1 || y0 < 0 || y1 >= hd.ny || y0 > y1 || z0 < 0 || z1 >= hd.nz || z0 > z1)
+            throw std::invalid_argument("region outside the domain");
+        // every GrainID of the run is an initial one (captures copy them) or 0
+        int gmin = 0, gmax = 0;
+        for (int32_t g : hd.gid) {
+            gmin = g < gmin ? g : gmin;
+            gmax = g > gmax ? g : gmax;
+        }
+        const uint32_t ngid = static_cast<uint32_t>(gmax - gmin + 1);
+        const size_t n = static_cast<size_t>(hd.ncell);
+        uint8_t *melted = nullptr;
+        uint32_t *gflag = nullptr;
+        unsigned long long *cnt = nullptr;
+        unsigned int *bins = nullptr;
+        struct Free {
+            std::vector<void *> p;
+            ~Free() {
+                for (void *q : p) (void)hipFree(q);
+            }
+        } tmp;
+        HIP_OK(hipMalloc(&melted, n));
+        tmp.p.push_back(melted);
+        HIP_OK(hipMalloc(&gflag, 4ull * ngid));
+        tmp.p.push_back(gflag);
+        HIP_OK(hipMalloc(&cnt, 8 * 4));
+        tmp.p.push_back(cnt);
+        HIP_OK(hipMalloc(&bins, 4ull * 2 * exaca::kNumOrientations));
+        tmp.p.push_back(bins);
+        HIP_OK(hipMemcpyAsync(melted, hd.melted.data(), n, hipMemcpyHostToDevice, h->stream));
+        HIP_OK(hipMemsetAsync(gflag, 0, 4ull * ngid, h->stream));
+        HIP_OK(hipMemsetAsync(cnt, 0, 8 * 4, h->stream));
+        HIP_OK(hipMemsetAsync(bins, 0, 4ull * 2 * exaca::kNumOrientations, h->stream));
+        const exaca::GrainRegion r{z0, z1, x0, x1, y0, y1};
+        exaca::launch_grain_stats(h->dv, melted, r, gmin, ngid, gflag, cnt, bins, bins + exaca::kNumOrientations,
+                                  h->stream);
+        HIP_OK(hipGetLastError());
+        std::vector<uint32_t> flags(ngid), ob(2 * exaca::kNumOrientations);
+        unsigned long long c[4];
+        HIP_OK(hipMemcpyAsync(flags.data(), gflag, 4ull * ngid, hipMemcpyDeviceToHost, h->stream));
+        HIP_OK(hipMemcpyAsync(ob.data(), bins, 4ull * ob.size(), hipMemcpyDeviceToHost, h->stream));
+        HIP_OK(hipMemcpyAsync(c, cnt, sizeof c, hipMemcpyDeviceToHost, h->stream));
+        HIP_OK(hipStreamSynchronize(h->stream));
+        if (c[2]) throw DeviceError("a GrainID outside the initial range");
+        long long grains = 0;
+        for (uint32_t f : flags) grains += f != 0;
+        // per orientation: the least angle of its three axes to +Z, capped at 62.7 degrees
+        // (analysis/src/GAprint.cpp:38-47: stored as float; src/CAprint.cpp:423-430: rounded)
+        long double sum = 0, sum_top = 0;
+        long long top = 0;
+        std::vector<int64_t> hb(64, 0);
+        for (int o = 0; o < exaca::kNumOrientations; o++) {
+            double amin = 62.7;
+            for (int ll = 0; ll < 3; ll++) {
+                const double a = std::abs((180 / M_PI) * std::acos(static_cast<double>(hd.guv[9 * o + 3 * ll + 2])));
+                if (a < amin) amin = a;
+            }
+            const float af = static_cast<float>(amin);
+            sum += static_cast<long double>(ob[o]) * af;
+            sum_top += static_cast<long double>(ob[exaca::kNumOrientations + o]) * af;
+            top += ob[exaca::kNumOrientations + o];
+            hb[static_cast<int>(std::round(amin))] += ob[o];
+        }
+        const long long vol = static_cast<long long>(x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 1);
+        const long long nmelt = static_cast<long long>(c[0]);
+        if (counts) {
+            counts[0] = vol;
+            counts[1] = nmelt;
+            counts[2] = grains;
+            counts[3] = static_cast<int64_t>(c[1]);
+            counts[4] = top;
+        }
+        if (values) {
+            values[0] = nmelt ? static_cast<double>(sum / nmelt) : 0.0;
+            values[1] = top ? static_cast<double>(sum_top / top) : 0.0;
+            values[2] = h->deck.deltax * h->deck.deltax * std::pow(10.0, 12) * (static_cast<double>(vol) / grains);
+            values[3] = static_cast<double>(c[1]) / static_cast<double>(vol);
+        }
+        if (hist) std::copy(hb.begin(), hb.end(), hist);
+    });
+}
+
 int exaca_query(const exaca_state *h, int64_t out[18]) {
     return guarded([&] {
         if (!h || !out) throw std::invalid_argument("null argument");
@@ -629,6 +717,10 @@ int exaca_host_state(const char *input_file, const char *data_dir, int field, vo
             break;
         case EXACA_FIELD_NUCLEI_LOC: put(hd.nloc.data(), hd.nloc.size()); break;
         case EXACA_FIELD_NUCLEI_TIME: put(hd.ntime.data(), hd.ntime.size()); break;
+        case EXACA_FIELD_MELTED:
+            t.assign(hd.melted.begin(), hd.melted.end());
+            put(t.data(), n);
+            break;
         default: throw std::invalid_argument("field not available from the host state");
         }
     });

@@ -92,7 +92,8 @@ float write_vtk_files(const Deck &d, const HostDomain &h, const std::vector<int3
         for (int n = 0; n < kNumOrientations; n++) {
             double amin = 62.7;
             for (int ll = 0; ll < 3; ll++) {
-                double a = std::abs((180 / M_PI) * std::acos(h.guv[9 * n + 3 * ll + 2]));
+                // the reference's unqualified acos of a float is the C double function
+                double a = std::abs((180 / M_PI) * std::acos(static_cast<double>(h.guv[9 * n + 3 * ll + 2])));
                 if (a < amin) amin = a;
             }
             rounded[n] = static_cast<int>(std::round(amin));

@@ -39,6 +39,7 @@ enum exaca_field {
     EXACA_FIELD_CELLTYPE_RAW = 6, /* uint8, type | 0x10 for cell-centred octahedra */
     EXACA_FIELD_NUCLEI_LOC = 7,   /* int32 per nucleation site (NucleiLocation), host state only */
     EXACA_FIELD_NUCLEI_TIME = 8,  /* int32 per nucleation site (NucleationTimes), host state only */
+    EXACA_FIELD_MELTED = 9,       /* int32 per cell, the reference's Melted flag, host state only */
 };
 
 /* Replaces the initialisation half of RunProgram_Reduced (src/runCA.cpp:37-256): reads the deck
@@ -81,6 +82,20 @@ int exaca_download(exaca_state *h, int field, void *dst, size_t dst_bytes);
  * frontier list length, frontier list capacity, device bytes allocated, critical-length
  * evaluations, liquid-neighbour masks built. */
 int exaca_query(const exaca_state *h, int64_t out[18]);
+
+/* Grain statistics of a region, computed on the device from the current GrainID field: the
+ * quantities of the reference's analysis executable (analysis/src/GAprint.cpp:17-108,
+ * PrintMisorientationData and the head of PrintSizeData) and its misorientation vtk
+ * (src/CAprint.cpp:419-432). region[6] = XMin, XMax, YMin, YMax, ZMin, ZMax, inclusive, in the
+ * whole-domain coordinates of the reference's vtk files (0 .. nx-1, 0 .. ny-1, 0 .. nz-1). A melted
+ * cell of GrainID 0 (a wall of a type-C domain) counts in orientation 0.
+ * counts[0..3]: cells in the region, melted cells, unique GrainIDs, cells of nucleated grains
+ * (GrainID < 0); counts[4]: melted cells in the top plane ZMax.
+ * values[0..3]: mean misorientation to +Z of the melted cells (degrees), the same in the top plane,
+ * mean grain volume (deltax^2 * 1e12 * region cells / grains, the reference's printed value),
+ * volume fraction of nucleated grains. hist[0..63]: melted cells per misorientation rounded to
+ * whole degrees (the misorientation vtk's values). Any output pointer may be NULL. */
+int exaca_grain_stats(exaca_state *h, const int32_t region[6], int64_t counts[5], double values[4], int64_t hist[64]);
 
 /* Releases the handle and its device memory. */
 int exaca_destroy(exaca_state *h);
