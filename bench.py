@@ -21,7 +21,8 @@ sample of the same workload: the first --cpu-cycles steps of the same deck cut t
 --cpu-z planes (the random streams are drawn plane by plane, so the substrate, nuclei and their
 times in those planes are the 800^3 deck's own); its CellCapture time as in the reference's log.
 
-ensemble: --ensemble process-map requests (directional solidification, cube sides log-uniform in
+ensemble: --ensemble process-map requests (256: the batched-ensemble configuration's instance
+count; directional solidification, cube sides log-uniform in
 [32, 256] cells, G, R and nucleation density log-uniform, one seed each, Poisson arrivals at
 50 req/s) solved three ways on the same GPU: all at once in the batched ensemble, served as they
 arrive (continuous batching under a 200 GiB budget), and one after another on the plain path.
@@ -121,7 +122,7 @@ def main() -> int:
     ap.add_argument("--cpu-z", type=int, default=40)
     ap.add_argument("--cpu-cycles", type=int, default=150)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--ensemble", type=int, default=64, help="process-map requests (0: skip)")
+    ap.add_argument("--ensemble", type=int, default=256, help="process-map requests (0: skip)")
     ap.add_argument("--copy-probe-gib", type=int, default=4, help="size of the HBM copy calibration (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
