@@ -54,7 +54,7 @@ constexpr uint16_t kFresh = 2;  // mask built by the cell's capture; not yet loo
 constexpr uint32_t kAlarmSlots = 64;
 // The apply pass: one workgroup per kApplyGroup capture-pass workgroups, each with a region of
 // kCandStage solidification candidates.
-constexpr uint32_t kApplyGroup = 4;
+constexpr uint32_t kApplyGroup = 2;
 constexpr uint32_t kCandStage = 2048;
 constexpr uint32_t apply_blocks(uint32_t claim_blocks) { return (claim_blocks + kApplyGroup - 1) / kApplyGroup; }
 // Words of one calendar slot: one bit per cell, rounded up to 32-byte rows (the list rebuild reads

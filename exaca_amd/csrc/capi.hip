@@ -150,10 +150,12 @@ void build_device_state(exaca_state *h) {
     HIP_OK(hipGetDevice(&dev));
     hipDeviceProp_t prop;
     HIP_OK(hipGetDeviceProperties(&prop, dev));
-    // Frontier kernels: up to 8 workgroups per CU, fewer for small domains (the frontier never
-    // exceeds the cell count, and a slice is at least one workgroup of cells).
-    // (at most 2048: the per-workgroup step counters are summed by one workgroup, 8 slots a thread)
-    const long long cap = std::min<long long>(2048, ((prop.multiProcessorCount * 8 + 7) / 8) * 8);
+    // Frontier kernels: up to 4 workgroups per CU (the capture pass is then resident in one round;
+    // measured at 800^3: 2, 3, 4, 6, 8 per CU give 110, 101, 96, 102, 118 us), fewer for small
+    // domains (the frontier never exceeds the cell count, and a slice is at least one workgroup of
+    // cells). (At most 2048: the per-workgroup step counters are summed by one workgroup, 8 slots
+    // a thread.)
+    const long long cap = std::min<long long>(2048, ((prop.multiProcessorCount * 4 + 7) / 8) * 8);
     const long long want = ((hd.ncell / 4096 + 7) / 8) * 8;
     h->blocks = static_cast<int>(std::max<long long>(8, std::min(cap, want)));
 
