@@ -405,6 +405,27 @@ __device__ __forceinline__ void put_part(const DomainView &d, int row, uint32_t 
     const unsigned long long t = block_sum(v, sh);
     if (threadIdx.x == 0) d.part[static_cast<size_t>(row) * d.nparts + b] = t;
 }
+// Four rows at once (one pair of barriers); sh holds 4 * kBlock / 64 entries.
+__device__ __forceinline__ void put_parts4(const DomainView &d, const int (&rows)[4], unsigned long long (&v)[4],
+                                           uint32_t b, unsigned long long *sh) {
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v[i] += __shfl_down(v[i], o, 64);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+        for (int i = 0; i < 4; i++) sh[i * (kBlock / 64) + w] = v[i];
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            unsigned long long t = 0;
+            for (int q = 0; q < kBlock / 64; q++) t += sh[i * (kBlock / 64) + q];
+            d.part[static_cast<size_t>(rows[i]) * d.nparts + b] = t;
+        }
+}
 
 // Growth index that an active cell's dl includes, at a step of growth index k. A cell not yet
 // looked at as an active cell (mask not yet known, or built by its capture: kFresh) holds its
@@ -432,7 +453,7 @@ __device__ __forceinline__ void solidify(const DomainView &d, const Growth &g, u
 // neighbour whose mask this empties and that is past its liquidus time finds no liquid neighbour in
 // this step's capture and turns solid here.
 __device__ __forceinline__ void nucleate_body(const DomainView &d, int p, int layer, uint32_t b, uint32_t nb) {
-    __shared__ unsigned long long sh[kBlock / 64];
+    __shared__ unsigned long long sh[4 * (kBlock / 64)];
     StepCtl *ctl = d.ctl;
     if (ctl->done) return;
     const int cycle = ctl->cycle[p];
@@ -486,10 +507,8 @@ __device__ __forceinline__ void nucleate_body(const DomainView &d, int p, int la
         else
             atomicOr(&ctl->error, 1);
     }
-    put_part(d, kPartNucNn, b, nn, sh);
-    put_part(d, kPartNucAct, b, act, sh);
-    put_part(d, kPartNucSol, b, n.nsol, sh);
-    put_part(d, kPartNucEntered, b, n.entered, sh);
+    unsigned long long v4[4] = {nn, act, n.nsol, n.entered};
+    put_parts4(d, {kPartNucNn, kPartNucAct, kPartNucSol, kPartNucEntered}, v4, b, sh);
 }
 
 // Critical lengths of the liquid neighbours of frontier cell j (half-diagonal dlj): claims on
@@ -556,7 +575,7 @@ __device__ uint32_t evaluate_claims(const DomainView &d, const Grid &gr, uint32_
 // list (an LDS counter, no global atomics), and the slice's count goes to slice_claims[b].
 
 __device__ __forceinline__ void claim_body(const DomainView &d, int p, int layer, uint32_t b, uint32_t nb) {
-    __shared__ unsigned long long sh[kBlock / 64];
+    __shared__ unsigned long long sh[4 * (kBlock / 64)];
     __shared__ uint32_t nclaim;
     __shared__ double gtab[kGrowStage];
     StepCtl *ctl = d.ctl;
@@ -637,10 +656,8 @@ __device__ __forceinline__ void claim_body(const DomainView &d, int p, int layer
     }
     __syncthreads();
     if (threadIdx.x == 0) d.slice_claims[b] = nclaim;
-    put_part(d, kPartClaimSol, b, n.nsol, sh);
-    put_part(d, kPartClaimEntered, b, n.entered, sh);
-    put_part(d, kPartClaimEval, b, n.neval, sh);
-    put_part(d, kPartClaimMask, b, n.nmask, sh);
+    unsigned long long v4[4] = {n.nsol, n.entered, n.neval, n.nmask};
+    put_parts4(d, {kPartClaimSol, kPartClaimEntered, kPartClaimEval, kPartClaimMask}, v4, b, sh);
 }
 
 // Pass B over the cells with claims. Workgroup a takes the claims of the kApplyGroup slices
@@ -748,15 +765,40 @@ __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t 
                 const Coord c = decode(jc, gr);
                 const Coord cqc{c.x + nb_dx(l), c.y + nb_dy(l), c.z + nb_dz(l)};
                 const uint32_t valid = valid_neighbours(cqc, gr.X, gr.Y, gr.nzA);
-                // the neighbours' masks are loaded before the octahedron is built (they arrive
-                // while it is computed)
-                // both words: the low word is the mask (or liquid marker), the high word of a liquid
-                // neighbour its claim word
+                // the neighbours' words: the low word is the mask (or liquid marker), the high word
+                // of a liquid neighbour its claim word
                 unsigned long long lw[26];
 #pragma unroll
                 for (int kk = 0; kk < 26; kk++)
                     lw[kk] = ((valid >> kk) & 1u) ? *reinterpret_cast<const unsigned long long *>(&d.cells[q + gr.off[kk]].lm)
                                                   : static_cast<unsigned long long>(kMaskUnknown);
+                // The new cell's mask: its liquid neighbours at its first visit are those liquid now
+                // and not claimed in this step (every claimed cell is captured in this step, and the
+                // claims are final). Until its store below, the new cell's own words read as a
+                // claimed liquid cell, which no other capture counts.
+                uint32_t liquid = 0, mw[26];
+#pragma unroll
+                for (int kk = 0; kk < 26; kk++) {
+                    liquid |= static_cast<uint32_t>(is_unclaimed_liquid(lw[kk])) << kk;
+                    mw[kk] = static_cast<uint32_t>(lw[kk]);
+                }
+                // updates only where the mask still holds q, all in flight together; their results
+                // are used after the octahedron is built, which hides their round trip
+                uint32_t need = 0;
+#pragma unroll
+                for (int kk = 0; kk < 26; kk++) {
+                    const uint32_t bit = 1u << kOpposite.v[kk];
+                    need |= static_cast<uint32_t>((mw[kk] & (kMaskUnknown | bit)) == bit) << kk;
+                }
+#pragma unroll
+                for (int kk = 0; kk < 26; kk++)
+                    if ((need >> kk) & 1u) mw[kk] = atomicAnd(mask_word(d, q + gr.off[kk]), ~(1u << kOpposite.v[kk]));
+                // a neighbour at or below the claimant's index still had q liquid at its visit
+                // in the reference's index order: its keep word gets this step's tag (every writer
+                // of a step writes the same value)
+#pragma unroll
+                for (int kk = 0; kk < 26; kk++)
+                    if (((need >> kk) & 1u) && q + gr.off[kk] <= jc) keep_word(d, q + gr.off[kk])[0] = serial + 1;
                 const int h = won_gid[lo];
                 int o = orientation_of(h);
                 if (o < 0) {
@@ -778,32 +820,6 @@ __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t 
                 ncap++;
                 qlo = q < qlo ? q : qlo;
                 qhi = q > qhi ? q : qhi;
-                // The new cell's mask: its liquid neighbours at its first visit are those liquid now
-                // and not claimed in this step (every claimed cell is captured in this step, and the
-                // claims are final). Until its store below, the new cell's own words read as a
-                // claimed liquid cell, which no other capture counts.
-                uint32_t liquid = 0, mw[26];
-#pragma unroll
-                for (int kk = 0; kk < 26; kk++) {
-                    liquid |= static_cast<uint32_t>(is_unclaimed_liquid(lw[kk])) << kk;
-                    mw[kk] = static_cast<uint32_t>(lw[kk]);
-                }
-                // updates only where the mask still holds q, all in flight together, then the results
-                uint32_t need = 0;
-#pragma unroll
-                for (int kk = 0; kk < 26; kk++) {
-                    const uint32_t bit = 1u << kOpposite.v[kk];
-                    need |= static_cast<uint32_t>((mw[kk] & (kMaskUnknown | bit)) == bit) << kk;
-                }
-#pragma unroll
-                for (int kk = 0; kk < 26; kk++)
-                    if ((need >> kk) & 1u) mw[kk] = atomicAnd(mask_word(d, q + gr.off[kk]), ~(1u << kOpposite.v[kk]));
-                // a neighbour at or below the claimant's index still had q liquid at its visit
-                // in the reference's index order: its keep word gets this step's tag (every writer
-                // of a step writes the same value)
-#pragma unroll
-                for (int kk = 0; kk < 26; kk++)
-                    if (((need >> kk) & 1u) && q + gr.off[kk] <= jc) keep_word(d, q + gr.off[kk])[0] = serial + 1;
 #pragma unroll
                 for (int kk = 0; kk < 26; kk++) {
                     const uint32_t bit = 1u << kOpposite.v[kk];
