@@ -932,12 +932,13 @@ __device__ __forceinline__ void settle_body(const DomainView &d, int p, int laye
 // count, then its inclusive prefix, as one tagged 64-bit word); then it writes the unit's cells.
 // A unit's count never waits on anything, and a unit waits only for lower units, of workgroups
 // dispatched before it or of earlier rounds of workgroups that are resident with it (the plain
-// launch has at most 257 workgroups); each wait is bounded all the same (error 8). The last
-// workgroup books the step meanwhile. (The range is a few thousand chunks at 800^3: a few hundred
-// units, about one per CU.)
+// launch has at most 513 workgroups, and at 3 workgroups per CU all are resident); each wait is
+// bounded all the same (error 8). The last workgroup books the step meanwhile. (The range is a few
+// thousand chunks at 800^3: several hundred units, one to two per CU; measured there, units of
+// 4, 8, 16 chunks with up to 512, 256, 128 workgroups: 23.9, 29.5, 65.9 us.)
 constexpr uint32_t kChunkWords = 8 * kBlock;  // thread t of a chunk holds its words 8t .. 8t + 7
 constexpr uint32_t kChunkCells = 32 * kChunkWords;
-constexpr uint32_t kUnitChunks = 8;
+constexpr uint32_t kUnitChunks = 4;
 constexpr uint32_t kRebuildStage = 8192;  // list entries of a unit staged in LDS (32 KB)
 constexpr unsigned long long kFlagCount = 1ull << 32;   // value = the chunk's own count
 constexpr unsigned long long kFlagPrefix = 2ull << 32;  // value = count of the chunk and all before it
@@ -1468,11 +1469,11 @@ void launch_nucleation(const DomainView &d, int parity, int layer, hipStream_t s
 void launch_claim(const DomainView &d, int parity, int layer, int blocks, hipStream_t s) {
     k_claim<<<blocks, kBlock, 0, s>>>(d, parity, layer);
 }
-// Workgroups of the list rebuild: at most one per unit of the whole domain and one per CU (each
+// Workgroups of the list rebuild: at most one per unit of the whole domain and two per CU (each
 // takes every (n-1)-th unit), plus the one that books the step.
 int rebuild_blocks(const DomainView &d, int blocks) {
     const uint32_t units = (d.nchunks + kUnitChunks - 1) / kUnitChunks;
-    const uint32_t cap = static_cast<uint32_t>(blocks < 256 ? blocks : 256);
+    const uint32_t cap = static_cast<uint32_t>(blocks < 512 ? blocks : 512);
     return static_cast<int>(units < cap ? units : cap) + 1;
 }
 void launch_apply(const DomainView &d, int parity, int layer, int blocks, hipStream_t s) {
