@@ -212,6 +212,7 @@ struct DomainView {
     uint32_t *alarms;       // kAlarmSlots bitmaps of one bit per cell: process the cell at the step
                             // whose serial is the bitmap's slot (mod kAlarmSlots)
     unsigned long long *chunk_flag;  // per unit of the list rebuild: tag | state | count or prefix
+    uint32_t rebuild_cap;   // list-rebuild workgroups the device holds at once (0: no limit)
     uint32_t nchunks;
     int32_t autostop;       // end the domain on the device at the first check with no liquid left
     long long liquid0;      // liquid cells of the layer at the start

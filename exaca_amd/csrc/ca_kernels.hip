@@ -1469,12 +1469,26 @@ void launch_nucleation(const DomainView &d, int parity, int layer, hipStream_t s
 void launch_claim(const DomainView &d, int parity, int layer, int blocks, hipStream_t s) {
     k_claim<<<blocks, kBlock, 0, s>>>(d, parity, layer);
 }
-// Workgroups of the list rebuild: at most one per unit of the whole domain and two per CU (each
-// takes every (n-1)-th unit), plus the one that books the step.
+// Workgroups of the list rebuild: at most one per unit of the whole domain, 512, and as many as
+// the device holds at once (each takes every (n-1)-th unit: a unit of a second round waits for a
+// unit of a higher workgroup of the first, which must then be resident), plus the one that books
+// the step.
 int rebuild_blocks(const DomainView &d, int blocks) {
     const uint32_t units = (d.nchunks + kUnitChunks - 1) / kUnitChunks;
-    const uint32_t cap = static_cast<uint32_t>(blocks < 512 ? blocks : 512);
+    uint32_t cap = static_cast<uint32_t>(blocks < 512 ? blocks : 512);
+    if (d.rebuild_cap && d.rebuild_cap < cap) cap = d.rebuild_cap;
     return static_cast<int>(units < cap ? units : cap) + 1;
+}
+uint32_t rebuild_resident(int cus) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_rebuild, kBlock, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    return static_cast<uint32_t>(per_cu) * static_cast<uint32_t>(cus);
+}
+uint32_t rebuild_units(long long ncell) {
+    const unsigned long long words = (static_cast<unsigned long long>(ncell) + 31) / 32;
+    const unsigned long long chunks = (words + kChunkWords - 1) / kChunkWords;
+    return static_cast<uint32_t>((chunks + kUnitChunks - 1) / kUnitChunks);
 }
 void launch_apply(const DomainView &d, int parity, int layer, int blocks, hipStream_t s) {
     const int cb = rebuild_blocks(d, blocks);
@@ -1513,9 +1527,9 @@ void launch_step_n(const DomainView *views, int slots, int parity, int layer, in
     k_claim_n<<<dim3(blocks, slots), kBlock, 0, s>>>(views, parity, layer);
     k_apply_n<<<dim3(apply_blocks(blocks), slots), kBlock, 0, s>>>(views, parity, blocks);
     k_settle_n<<<dim3(settle_blocks(blocks), slots), kBlock, 0, s>>>(views, parity, layer);
-    // ensemble domains are small (a few dozen units at most: one round of 64 workgroups, each
-    // waiting only for workgroups dispatched before it) plus the booking workgroup
-    k_rebuild_n<<<dim3((blocks < 64 ? blocks : 64) + 1, slots), kBlock, 0, s>>>(views, parity, blocks, nuc_blocks);
+    // one round of workgroups covers every unit of an ensemble domain (at most kEnsembleUnits):
+    // each waits only for workgroups of its row dispatched before it; plus the booking workgroup
+    k_rebuild_n<<<dim3(kEnsembleUnits + 1, slots), kBlock, 0, s>>>(views, parity, blocks, nuc_blocks);
 }
 void launch_check(const DomainView &d, int cycle, int layer, CheckResult *out, hipStream_t s) {
     k_check<<<grid_for(d.ncell, 4096), kBlock, 0, s>>>(d, cycle, layer, out);

@@ -57,6 +57,12 @@ struct GrainRegion {
 void launch_grain_stats(const DomainView &d, const uint8_t *melted, const GrainRegion &r, int gmin, uint32_t ngid,
                         uint32_t *gflag, unsigned long long *counts, unsigned int *obins, unsigned int *otop,
                         hipStream_t s);
+// The list rebuild: workgroups the device holds at once (its occupancy on cus CUs), and the units
+// of a domain of ncell cells. An ensemble domain has at most kEnsembleUnits units, so that one
+// round of workgroups covers it.
+uint32_t rebuild_resident(int cus);
+uint32_t rebuild_units(long long ncell);
+constexpr uint32_t kEnsembleUnits = 128;
 // Plain copy of n 16-byte words (HBM calibration).
 void launch_copy(const uint4 *src, uint4 *dst, size_t n, hipStream_t s);
 // Device-side self test of the capture geometry against host values: one octahedron per input row.

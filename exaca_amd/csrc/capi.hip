@@ -182,6 +182,7 @@ void build_device_state(exaca_state *h) {
     d.nchunks = static_cast<uint32_t>((nwords + 2047) / 2048);
     d.alarms = h->alloc<uint32_t>(kAlarmSlots * alarm_stride(d.ncell));
     d.chunk_flag = h->alloc<unsigned long long>(d.nchunks);
+    d.rebuild_cap = exaca::rebuild_resident(prop.multiProcessorCount);
     d.nnuclei = static_cast<int32_t>(hd.nloc.size());
     int32_t *nloc = h->alloc<int32_t>(hd.nloc.size());
     int32_t *ntime = h->alloc<int32_t>(hd.ntime.size());

@@ -305,6 +305,10 @@ int exaca_prepare(const char *input_file, const char *data_dir, uint64_t seed, e
         const exaca::HostDomain &h = pr->host;
         pr->grow = exaca::growth_table(h.A, h.B, h.C, h.D, h.dU, 1 << 25);
         if (pr->grow.size() >= (1u << 25)) throw std::runtime_error("undercooling never saturates within 2^25 steps");
+        if (exaca::rebuild_units(h.ncell) > exaca::kEnsembleUnits)
+            throw std::invalid_argument("domain too large for an ensemble slot (at most " +
+                                        std::to_string(exaca::kEnsembleUnits) +
+                                        " list-rebuild units, about 33 M cells); run it on the plain path");
         for (long long q = 0; q < h.ncell; q++) pr->liquid0 += (h.ct[q] == exaca::kLiquid && h.lid[q] == 0);
         pr->bytes = domain_bytes(h, pr->grow.size());
         *out = pr.release();
