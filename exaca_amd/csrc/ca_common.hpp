@@ -198,7 +198,7 @@ struct DomainView {
     uint32_t list_cap;
     int32_t nnuclei;
     const int32_t *nloc;
-    const int32_t *ntime;
+    int32_t *ntime;         // NucleationTimes; a site examined past its time is set to INT32_MAX
     const float *guv;       // 10000 x 9 unit-vector components (GrainUnitVector)
     const float *seed_cdl;  // 10000 x 26 critical diagonal lengths of cell-centred octahedra
     const double *grow;     // growth increment after k executed steps below the liquidus

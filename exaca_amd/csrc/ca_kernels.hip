@@ -467,8 +467,14 @@ __device__ __forceinline__ void nucleate_body(const DomainView &d, int p, int la
     unsigned long long nn = 0, act = 0;
     StepCounts n{0, 0, 0, 0};
     for (int i = b * kBlock + threadIdx.x; i < d.nnuclei; i += nb * kBlock) {
+        if (cycle < d.ntime[i]) continue;
         const uint32_t q = static_cast<uint32_t>(d.nloc[i]);
-        if (!(cycle >= d.ntime[i] && d.cells[q].ct == kLiquid && d.cells[q].lid <= layer)) continue;
+        const uint32_t tail = reinterpret_cast<const uint32_t *>(d.cells + q)[7];  // lid | ct << 8 | flags
+        if (static_cast<int8_t>(tail & 0xFFu) > layer) continue;
+        // Past its time and in this layer, the site is decided now: a cell that is not liquid
+        // never turns liquid again, so the site is not looked at after this step.
+        d.ntime[i] = 0x7FFFFFFF;
+        if (((tail >> 8) & 0xFFu) != kLiquid) continue;
         const Coord c = decode(q, gr);
         if (c.x > 0 && c.x < d.X - 1 && c.y > 0 && c.y < d.Y - 1) nn++;
         Cell &cq = d.cells[q];
