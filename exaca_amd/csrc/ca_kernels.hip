@@ -595,12 +595,30 @@ __device__ __forceinline__ void claim_body(const DomainView &d, int p, int layer
     const uint32_t *list = d.list[p];
     const Grid gr = grid_of(d);
     StepCounts n{0, 0, 0, 0};
+    // software pipeline: the list entry two rounds ahead and the record one round ahead are loaded
+    // while the current round is processed
+    uint32_t jn = 0, jnn = 0;
+    CellWords wn{};
+    {
+        const uint32_t i1 = begin + threadIdx.x, i2 = i1 + kBlock;
+        if (i1 < end) jn = list[i1];
+        if (i2 < end) jnn = list[i2];
+        if (i1 < end) wn = load_cell(d.cells + jn);
+    }
     for (uint32_t i0 = begin; i0 < end; i0 += kBlock) {
         const uint32_t i = i0 + threadIdx.x;
         uint32_t j = 0, claimed = 0;
+        const uint32_t jc = jn;
+        const CellWords wc = wn;
+        {
+            const uint32_t i3 = i + 2 * kBlock;
+            jn = jnn;
+            if (i + kBlock < end) wn = load_cell(d.cells + jn);
+            if (i3 < end) jnn = list[i3];
+        }
         if (i < end) {
-            j = list[i];
-            const CellWords w = load_cell(d.cells + j);
+            j = jc;
+            const CellWords w = wc;
             if ((w.ct & kTypeMask) == kActive) {
                 const bool waiting = !(cycle > w.cts && w.lid <= layer);
                 const int k = growth_index(d, w.cts, cycle);
