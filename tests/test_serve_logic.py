@@ -119,3 +119,15 @@ def test_digest_identifies_results():
     b = np.arange(10) * 3
     b[4] += 1
     assert a != serve.digest(5000, np.arange(10), b)
+
+
+def test_ensemble_refuses_domains_beyond_one_rebuild_round(tmp_path):
+    """An ensemble slot covers its domain's list rebuild in one round of workgroups (at most 128
+    units of 4 x 65536 cells): a larger domain is refused at preparation, on the host."""
+    from conftest import c_deck
+
+    ok = ea.Prepared(c_deck(tmp_path / "ok.txt", 256, density=1))
+    assert ok.cells == 260 * 260 * 258  # walls and the ghost ring in x, y; walls in z
+    ok.close()
+    with pytest.raises(ea.ExacaError, match="too large for an ensemble"):
+        ea.Prepared(c_deck(tmp_path / "big.txt", 330, density=1))
