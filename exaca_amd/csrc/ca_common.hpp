@@ -111,6 +111,7 @@ struct StepCtl {
     int32_t done;         // set by the device when a domain with autostop finishes
     int32_t final_cycle;  // the check cycle at which it finished
     int32_t last_cycle;   // the last executed step
+    int32_t next_nuc[2];  // least NucleationTime of the sites not yet decided, for the step of parity p
     unsigned long long nactive;     // active cells at the end of the last step
     unsigned long long nonwaiting;  // of those, past their liquidus time (the reference's numSteer)
     int32_t jump_from[kMaxJumps];  // cycle J before the jump; cycles J+1 .. T-1 were skipped

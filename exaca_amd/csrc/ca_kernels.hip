@@ -452,25 +452,47 @@ __device__ __forceinline__ void solidify(const DomainView &d, const Growth &g, u
 // Nucleation (src/CAupdate.cpp:67-200). A nucleus leaves the liquid masks of its neighbours; a
 // neighbour whose mask this empties and that is past its liquidus time finds no liquid neighbour in
 // this step's capture and turns solid here.
+// Most steps have no site due: next_nuc[p], the least time of the sites not yet decided, lets every
+// workgroup return at once (its counter rows zero). A step that examines sites sets the next step's
+// value from what is left (next_nuc[1-p], reset to INT32_MAX by the capture pass of the step before),
+// a step that does not passes its own on.
+constexpr int kPartNucRows[4] = {kPartNucNn, kPartNucAct, kPartNucSol, kPartNucEntered};
 __device__ __forceinline__ void nucleate_body(const DomainView &d, int p, int layer, uint32_t b, uint32_t nb) {
     __shared__ unsigned long long sh[4 * (kBlock / 64)];
+    __shared__ int smin;
     StepCtl *ctl = d.ctl;
     if (ctl->done) return;
     const int cycle = ctl->cycle[p];
+    const int gate = ctl->next_nuc[p];
     if (b == 0 && threadIdx.x == 0) {
         ctl->cycle[1 - p] = cycle + 1;
         ctl->serial[1 - p] = ctl->serial[p] + 1;
         ctl->ncand = 0;
         ctl->rebuild_seq = ctl->rebuild_seq + 1;
+        if (cycle < gate) ctl->next_nuc[1 - p] = gate;
     }
+    if (cycle < gate) {
+        if (threadIdx.x < 4) d.part[static_cast<size_t>(kPartNucRows[threadIdx.x]) * d.nparts + b] = 0;
+        return;
+    }
+    if (threadIdx.x == 0) smin = 0x7FFFFFFF;
+    __syncthreads();
+    int tmin = 0x7FFFFFFF;  // least time of this thread's sites left undecided
     const Grid gr = grid_of(d);
     unsigned long long nn = 0, act = 0;
     StepCounts n{0, 0, 0, 0};
     for (int i = b * kBlock + threadIdx.x; i < d.nnuclei; i += nb * kBlock) {
-        if (cycle < d.ntime[i]) continue;
+        const int t = d.ntime[i];
+        if (cycle < t) {
+            tmin = t < tmin ? t : tmin;
+            continue;
+        }
         const uint32_t q = static_cast<uint32_t>(d.nloc[i]);
         const uint32_t tail = reinterpret_cast<const uint32_t *>(d.cells + q)[7];  // lid | ct << 8 | flags
-        if (static_cast<int8_t>(tail & 0xFFu) > layer) continue;
+        if (static_cast<int8_t>(tail & 0xFFu) > layer) {
+            tmin = t < tmin ? t : tmin;
+            continue;
+        }
         // Past its time and in this layer, the site is decided now: a cell that is not liquid
         // never turns liquid again, so the site is not looked at after this step.
         d.ntime[i] = 0x7FFFFFFF;
@@ -513,8 +535,15 @@ __device__ __forceinline__ void nucleate_body(const DomainView &d, int p, int la
         else
             atomicOr(&ctl->error, 1);
     }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const int v = __shfl_xor(tmin, o, 64);
+        tmin = v < tmin ? v : tmin;
+    }
+    if ((threadIdx.x & 63) == 0) atomicMin(&smin, tmin);
     unsigned long long v4[4] = {nn, act, n.nsol, n.entered};
     put_parts4(d, {kPartNucNn, kPartNucAct, kPartNucSol, kPartNucEntered}, v4, b, sh);
+    if (threadIdx.x == 0 && smin != 0x7FFFFFFF) atomicMin(&ctl->next_nuc[1 - p], smin);
 }
 
 // Critical lengths of the liquid neighbours of frontier cell j (half-diagonal dlj): claims on
@@ -587,6 +616,8 @@ __device__ __forceinline__ void claim_body(const DomainView &d, int p, int layer
     StepCtl *ctl = d.ctl;
     if (ctl->done) return;
     if (threadIdx.x == 0) nclaim = 0;
+    // this step's nucleation read its gate: reset it for the nucleation two steps on
+    if (b == 0 && threadIdx.x == 0) ctl->next_nuc[p] = 0x7FFFFFFF;
     const Growth g = staged_growth(d, gtab);
     const int cycle = ctl->cycle[p];
     const uint32_t serial = ctl->serial[p];

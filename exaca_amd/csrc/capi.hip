@@ -42,6 +42,9 @@ StepCtl initial_ctl(const HostDomain &hd) {
         c.ehi = std::max(c.ehi, q);
     }
     c.nactive = hd.active.size();
+    c.next_nuc[0] = 0x7FFFFFFF;
+    c.next_nuc[1] = 0x7FFFFFFF;
+    for (int32_t t : hd.ntime) c.next_nuc[0] = std::min(c.next_nuc[0], t);
     return c;
 }
 
