@@ -3,11 +3,15 @@
 // z = 1 .. nz-2 and zero on the bottom and top planes (src/CAprint.cpp:18-112).
 #include "output.hpp"
 
+#include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <fstream>
 #include <iostream>
+#include <memory>
 #include <sstream>
+#include <thread>
 
 #include "ca_common.hpp"
 
@@ -18,6 +22,21 @@ template <typename T>
 int gathered(const HostDomain &h, const std::vector<T> &f, int k, int i, int j) {
     if (k < 1 || k > h.nz - 2) return 0;
     return static_cast<int>(f[(static_cast<long long>(k) * h.X + (i + 1)) * h.Y + (j + 1)]);
+}
+
+// "<v> " at p; returns the end. At most 12 bytes.
+inline char *put_num(char *p, int v) {
+    char tmp[12];
+    int L = 0;
+    unsigned u = v < 0 ? 0u - static_cast<unsigned>(v) : static_cast<unsigned>(v);
+    do {
+        tmp[L++] = static_cast<char>('0' + u % 10);
+        u /= 10;
+    } while (u);
+    if (v < 0) *p++ = '-';
+    while (L) *p++ = tmp[--L];
+    *p++ = ' ';
+    return p;
 }
 
 // Buffered writer: ASCII vtk of a large domain is dominated by formatting, so integers are
@@ -46,24 +65,72 @@ struct Out {
         std::copy(s.begin(), s.end(), buf.begin() + n);
         n += s.size();
     }
-    void num_space(int v) {  // "<v> "
-        if (n + 16 > buf.size()) flush();
-        char tmp[16];
-        int L = 0;
-        unsigned u = v < 0 ? 0u - static_cast<unsigned>(v) : static_cast<unsigned>(v);
-        do {
-            tmp[L++] = static_cast<char>('0' + u % 10);
-            u /= 10;
-        } while (u);
-        if (v < 0) buf[n++] = '-';
-        while (L) buf[n++] = tmp[--L];
-        buf[n++] = ' ';
-    }
-    void nl() {
-        if (n + 1 > buf.size()) flush();
-        buf[n++] = '\n';
+    void bytes(const char *p, size_t len) {
+        flush();
+        std::fwrite(p, 1, len, f);
     }
 };
+
+// Text buffers of the plane formatter, one per worker, kept for every field of a run so that
+// their pages are touched once.
+struct PlaneBuffers {
+    int nt = static_cast<int>(std::min(16u, std::max(1u, std::thread::hardware_concurrency())));
+    size_t size = 0;
+    std::vector<std::unique_ptr<char[]>> buf;  // left uninitialised: only written bytes are read
+    char *get(int t, size_t need) {
+        if (need > size) {
+            buf.clear();
+            for (int i = 0; i < nt; i++) buf.emplace_back(new char[need]);
+            size = need;
+        }
+        return buf[t].get();
+    }
+};
+
+// Writes planes k0 .. k1-1 of a field, one line per plane of `cols` numbers. Up to 16 workers
+// format the planes of a batch into their own buffers at once (about 4 MB of text each, at least
+// one plane), and the buffers go to the file in plane order. Each plane's numbers come from
+// plane(k, put), put(v) appending "<v> ". Returns the sums over planes of the two counters plane()
+// returns (the misorientation writer's melted and nucleated cells).
+using Counts = std::array<long long, 2>;
+template <typename F>
+Counts write_planes(Out &o, PlaneBuffers &pb, int k0, int k1, long long cols, F plane) {
+    const int nk = k1 - k0;
+    if (nk <= 0) return {0, 0};
+    const size_t per_plane = static_cast<size_t>(cols) * 12 + 1;
+    const int nt = std::min(pb.nt, nk);
+    const int pp = std::min(static_cast<int>(std::max<size_t>(1, (4u << 20) / per_plane)), (nk + nt - 1) / nt);
+    for (int t = 0; t < nt; t++) pb.get(t, per_plane * pp);
+    std::vector<size_t> len(nt);
+    std::vector<Counts> acc(nt);
+    Counts total{0, 0};
+    for (int b0 = k0; b0 < k1; b0 += nt * pp) {
+        auto work = [&](int t) {
+            char *const p0 = pb.buf[t].get();
+            char *p = p0;
+            Counts a{0, 0};
+            const int s = b0 + t * pp, e = std::min(k1, s + pp);
+            for (int k = s; k < e; k++) {
+                const Counts c = plane(k, [&p](int v) { p = put_num(p, v); });
+                a[0] += c[0];
+                a[1] += c[1];
+                *p++ = '\n';
+            }
+            len[t] = static_cast<size_t>(p - p0);
+            acc[t] = a;
+        };
+        std::vector<std::thread> th;
+        for (int t = 1; t < nt; t++) th.emplace_back(work, t);
+        work(0);
+        for (auto &x : th) x.join();
+        for (int t = 0; t < nt; t++) {
+            o.bytes(pb.buf[t].get(), len[t]);
+            total[0] += acc[t][0];
+            total[1] += acc[t][1];
+        }
+    }
+    return total;
+}
 
 std::string header(int nx, int ny, int nz) {
     std::ostringstream o;
@@ -82,6 +149,7 @@ int orientation_index(int gid) {
 
 float write_vtk_files(const Deck &d, const HostDomain &h, const std::vector<int32_t> &gid) {
     float fraction = 0.0f;
+    PlaneBuffers pb;
     if (d.print_misorientation) {
         std::string fname = d.path_to_output + d.output_base + "_Misorientations.vtk";
         std::cout << "Printing Paraview file of grain misorientations" << std::endl;
@@ -98,25 +166,27 @@ float write_vtk_files(const Deck &d, const HostDomain &h, const std::vector<int3
             }
             rounded[n] = static_cast<int>(std::round(amin));
         }
-        long long nucleated = 0, melted = 0;
-        for (int k = 1; k < h.nz - 1; k++) {
+        const Counts mn = write_planes(o, pb, 1, h.nz - 1, static_cast<long long>(h.nx - 2) * (h.ny - 2),
+                                              [&](int k, auto put) {
+            long long m = 0, nuc = 0;
             for (int j = 1; j < h.ny - 1; j++)
                 for (int i = 1; i < h.nx - 1; i++) {
                     if (gathered(h, h.melted, k, i, j) == 0) {
-                        o.num_space(200);
+                        put(200);
                         continue;
                     }
-                    melted++;
+                    m++;
                     int g = gathered(h, gid, k, i, j);
                     int a = rounded[orientation_index(g)];
                     if (g < 0) {
                         a += 100;
-                        nucleated++;
+                        nuc++;
                     }
-                    o.num_space(a);
+                    put(a);
                 }
-            o.nl();
-        }
+            return Counts{m, nuc};
+        });
+        const long long melted = mn[0], nucleated = mn[1];
         fraction = static_cast<float>(nucleated) / static_cast<float>(melted);
         std::cout << "Volume fraction of solidified portion of domain claimed by nucleated grains: " << fraction
                   << std::endl;
@@ -128,24 +198,25 @@ float write_vtk_files(const Deck &d, const HostDomain &h, const std::vector<int3
         // Without full output the reference writes only LayerID from a view it never gathered
         // (empty); zeros are written in its place.
         o.str("SCALARS LayerID int 1\nLOOKUP_TABLE default\n");
-        for (int k = 0; k < h.nz; k++) {
+        const long long cols = static_cast<long long>(h.nx) * h.ny;
+        write_planes(o, pb, 0, h.nz, cols, [&](int k, auto put) {
             for (int j = 0; j < h.ny; j++)
-                for (int i = 0; i < h.nx; i++) o.num_space(d.print_full ? gathered(h, h.lid, k, i, j) : 0);
-            o.nl();
-        }
+                for (int i = 0; i < h.nx; i++) put(d.print_full ? gathered(h, h.lid, k, i, j) : 0);
+            return Counts{0, 0};
+        });
         if (d.print_full) {
             o.str("SCALARS GrainID int 1\nLOOKUP_TABLE default\n");
-            for (int k = 0; k < h.nz; k++) {
+            write_planes(o, pb, 0, h.nz, cols, [&](int k, auto put) {
                 for (int j = 0; j < h.ny; j++)
-                    for (int i = 0; i < h.nx; i++) o.num_space(gathered(h, gid, k, i, j));
-                o.nl();
-            }
+                    for (int i = 0; i < h.nx; i++) put(gathered(h, gid, k, i, j));
+                return Counts{0, 0};
+            });
             o.str("SCALARS Melted int 1\nLOOKUP_TABLE default\n");
-            for (int k = 0; k < h.nz; k++) {
+            write_planes(o, pb, 0, h.nz, cols, [&](int k, auto put) {
                 for (int j = 0; j < h.ny; j++)
-                    for (int i = 0; i < h.nx; i++) o.num_space(gathered(h, h.melted, k, i, j) ? 1 : 0);
-                o.nl();
-            }
+                    for (int i = 0; i < h.nx; i++) put(gathered(h, h.melted, k, i, j) ? 1 : 0);
+                return Counts{0, 0};
+            });
         }
     }
     return fraction;
