@@ -24,6 +24,95 @@ inline long long cell(const HostDomain &h, int k, int i, int j) {
     return (static_cast<long long>(k) * h.X + i) * h.Y + j;
 }
 
+// Runs f(k) for k = k0 .. k1-1 on up to 16 threads (f must touch only plane k's cells).
+template <typename F>
+void for_planes(int k0, int k1, F f) {
+    const int n = k1 - k0;
+    if (n <= 0) return;
+    const int nt = std::min<int>(n, static_cast<int>(std::min(16u, std::max(1u, std::thread::hardware_concurrency()))));
+    std::atomic<int> next{k0};
+    auto work = [&] {
+        for (int k; (k = next.fetch_add(1)) < k1;) f(k);
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; t++) th.emplace_back(work);
+    work();
+    for (auto &x : th) x.join();
+}
+
+// The 64-bit Mersenne Twister with the parameters the C++ standard fixes for std::mt19937_64
+// ([rand.predef]; Matsumoto and Nishimura's MT19937-64), refilled 312 outputs at a time: the
+// same sequence as std::mt19937_64 with the same seed, without its per-call bookkeeping.
+class Mt64Block {
+  public:
+    static constexpr int kN = 312, kM = 156;
+    explicit Mt64Block(uint64_t seed) {
+        s_[0] = seed;
+        for (int i = 1; i < kN; i++) s_[i] = 6364136223846793005ULL * (s_[i - 1] ^ (s_[i - 1] >> 62)) + i;
+    }
+    // the next 312 outputs
+    const uint64_t *refill() {
+        constexpr uint64_t kUpper = ~0ULL << 31, kLower = (1ULL << 31) - 1, kA = 0xB5026F5AA96619E9ULL;
+        auto mix = [&](uint64_t hi, uint64_t lo, uint64_t far) {
+            const uint64_t y = (hi & kUpper) | (lo & kLower);
+            return far ^ (y >> 1) ^ ((y & 1) ? kA : 0);
+        };
+        for (int i = 0; i < kN - kM; i++) s_[i] = mix(s_[i], s_[i + 1], s_[i + kM]);
+        for (int i = kN - kM; i < kN - 1; i++) s_[i] = mix(s_[i], s_[i + 1], s_[i + kM - kN]);
+        s_[kN - 1] = mix(s_[kN - 1], s_[0], s_[kM - 1]);
+        for (int i = 0; i < kN; i++) {
+            uint64_t z = s_[i];
+            z ^= (z >> 29) & 0x5555555555555555ULL;
+            z ^= (z << 17) & 0x71D67FFFEDA60000ULL;
+            z ^= (z << 37) & 0xFFF7EEE000000000ULL;
+            out_[i] = z ^ (z >> 43);
+        }
+        return out_;
+    }
+
+  private:
+    uint64_t s_[kN], out_[kN];
+};
+
+// libstdc++'s uniform_real_distribution<double>(0, 1) over a 64-bit engine draws
+// generate_canonical with one engine call: double(x) * 2^-64, clamped below 1. That is
+// nondecreasing in x, so `draw < v` holds exactly for x below the cut returned here (all x when
+// `all` is set).
+uint64_t draw_cut(double v, bool &all) {
+    auto below = [v](uint64_t x) {
+        double u = static_cast<double>(x) * 0x1p-64;
+        if (u >= 1.0) u = std::nextafter(1.0, 0.0);
+        return u < v;
+    };
+    all = below(~0ULL);
+    if (all) return 0;
+    uint64_t lo = 0, hi = ~0ULL;  // below(hi) is false; find the least such x
+    while (lo < hi) {
+        const uint64_t mid = lo + (hi - lo) / 2;
+        if (below(mid))
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Indices (in draw order) of the draws below v among the first n draws of
+// uniform_real_distribution<double>(0, 1) over mt19937_64(seed).
+std::vector<long long> draws_below(uint64_t seed, double v, long long n) {
+    bool all = false;
+    const uint64_t cut = draw_cut(v, all);
+    std::vector<long long> hits;
+    Mt64Block g(seed);
+    for (long long base = 0; base < n; base += Mt64Block::kN) {
+        const uint64_t *x = g.refill();
+        const int m = static_cast<int>(std::min<long long>(Mt64Block::kN, n - base));
+        for (int i = 0; i < m; i++)
+            if (all || x[i] < cut) hits.push_back(base + i);
+    }
+    return hits;
+}
+
 // FindItBounds-equivalent neighbour filter: a neighbour exists when it stays inside the rank's
 // X-Y slab (src/CAfunctions.cpp:169-213).
 inline bool in_slab(const HostDomain &h, int i, int j, int n) {
@@ -37,7 +126,7 @@ void temperature_directional(const Deck &d, HostDomain &h) {
     h.zhigh = h.nz - 1;
     h.nzA = h.nz;
     h.dU = static_cast<float>(d.R * d.deltat);
-    for (int k = 0; k < h.nz; k++) {
+    for_planes(0, h.nz, [&](int k) {
         int crit = static_cast<int>(((k - 1) * d.G * d.deltax) / (d.R * d.deltat));
         for (int i = 0; i < h.X; i++)
             for (int j = 0; j < h.Y; j++) {
@@ -47,7 +136,7 @@ void temperature_directional(const Deck &d, HostDomain &h) {
                 int gx = i + h.xoff, gy = j + h.yoff;
                 h.melted[p] = (gx > -1 && gx < h.nx && gy > -1 && gy < h.ny && k > 0 && k < h.nz - 1);
             }
-    }
+    });
 }
 
 // TempInit_SpotMelt (src/CAinitialize.cpp:1058-1124). Cells outside every spot keep
@@ -244,19 +333,45 @@ void walls_and_interface(HostDomain &h) {
             }
 }
 
-// GrainInit (src/CAinitialize.cpp:1834-2206), first layer of rank 0 of 1: potential nuclei drawn
-// from mt19937_64(0) over cells with melting data in (z, y, x) order, numbered -1, -2, ... in
-// index order; then liquid cells with no non-solid, non-wall neighbour turn solid, in index order.
-void grains(const Deck &d, HostDomain &h) {
-    std::mt19937_64 gen(d.seed);
-    std::uniform_real_distribution<double> dis(0.0, 1.0);
-    const double bulk = d.nmax * d.deltax * d.deltax * d.deltax;
-    for (int k = 1; k < h.nz - 1; k++)
+// Draws of GrainInit's potential nuclei (src/CAinitialize.cpp:1834-2206): one per interior cell
+// with melting data, in (z, y, x) order, from mt19937_64(seed). first[k] is the number of draws
+// before plane k (k = 1 .. nz-1); the draws themselves need nothing else, so they can run while
+// the substrate is built.
+struct NucleusDraws {
+    std::vector<long long> first, hits;
+};
+std::vector<long long> draws_before_planes(const HostDomain &h) {
+    std::vector<long long> n(h.nz + 1, 0);
+    for_planes(1, h.nz - 1, [&](int k) {
+        long long c = 0;
+        for (int j = 1; j < h.Y - 1; j++)
+            for (int i = 1; i < h.X - 1; i++) c += h.cts[cell(h, k, i, j)] != 0;
+        n[k + 1] = c;
+    });
+    n[1] = 0;
+    for (int k = 2; k <= h.nz; k++) n[k] += n[k - 1];
+    return n;
+}
+
+// GrainInit (src/CAinitialize.cpp:1834-2206), first layer of rank 0 of 1: the cells whose draw is
+// below the bulk nucleation probability are potential nuclei, numbered -1, -2, ... in index
+// order; then liquid cells with no non-solid, non-wall neighbour turn solid, in index order.
+void grains(const Deck &d, HostDomain &h, const NucleusDraws &nd) {
+    size_t next = 0;
+    for (int k = 1; k < h.nz - 1 && next < nd.hits.size(); k++) {
+        if (nd.hits[next] >= nd.first[k + 1]) continue;
+        long long c = nd.first[k];
         for (int j = 1; j < h.Y - 1; j++)
             for (int i = 1; i < h.X - 1; i++) {
                 long long p = cell(h, k, i, j);
-                if (h.cts[p] != 0 && dis(gen) < bulk) h.ct[p] = kTemporaryInit;
+                if (h.cts[p] == 0) continue;
+                if (next < nd.hits.size() && nd.hits[next] == c) {
+                    h.ct[p] = kTemporaryInit;
+                    next++;
+                }
+                c++;
             }
+    }
     int ncounter = -1;
     const long long lo = static_cast<long long>(h.X) * h.Y, hi = static_cast<long long>(h.X) * h.Y * (h.nz - 1);
     for (long long p = lo; p < hi; p++) {
@@ -381,18 +496,30 @@ HostDomain initialize(const Deck &d, bool verbose) {
         temperature_directional(d, h);
     else
         temperature_spots(d, h);
-    h.guv = read_orientations(d.orientation_file);
-    if (d.sim_type == "C") {
-        substrate_directional(d, h);
+    // GrainInit's draws on their own thread while the substrate is built (it reads only
+    // CritTimeStep, which is final here; the substrate writes only CellType and GrainID)
+    NucleusDraws nd;
+    nd.first = draws_before_planes(h);
+    const double bulk = d.nmax * d.deltax * d.deltax * d.deltax;
+    std::thread drawer([&] { nd.hits = draws_below(d.seed, bulk, nd.first[std::max(h.nz - 1, 1)]); });
+    try {
+        h.guv = read_orientations(d.orientation_file);
+        if (d.sim_type == "C") {
+            substrate_directional(d, h);
+        }
+        else {
+            if (d.use_substrate_file)
+                substrate_file(d, h);
+            else
+                substrate_spacing(d, h);
+            walls_and_interface(h);
+        }
+    } catch (...) {
+        drawer.join();
+        throw;
     }
-    else {
-        if (d.use_substrate_file)
-            substrate_file(d, h);
-        else
-            substrate_spacing(d, h);
-        walls_and_interface(h);
-    }
-    grains(d, h);
+    drawer.join();
+    grains(d, h, nd);
     nuclei(d, h);
     h.A = d.A * d.deltat / d.deltax;
     h.B = d.B * d.deltat / d.deltax;

@@ -24,6 +24,10 @@ DECKS = {
     "ragged": lambda p: c_deck(p / "r.txt", 17, nx=29, ny=11, density=2000, fraction=0.2),
     "spots_3x2": lambda p: s_deck(p / "s.txt", nsx=3, nsy=2, offset=18, radius=11, density=500, spacing=9),
     "empty_substrate": lambda p: c_deck(p / "e.txt", 9, density=0, fraction=0.0),
+    # every third draw a nucleus site, and every draw one (probability above 1): the product's
+    # block generator and integer cut must reproduce <random>'s decisions draw for draw
+    "dense_nuclei": lambda p: c_deck(p / "n.txt", 21, nx=19, ny=23, density=300000, fraction=0.1),
+    "all_nuclei": lambda p: c_deck(p / "a.txt", 8, density=2000000, fraction=0.1),
 }
 
 
