@@ -55,7 +55,7 @@ class Mt64Block {
         constexpr uint64_t kUpper = ~0ULL << 31, kLower = (1ULL << 31) - 1, kA = 0xB5026F5AA96619E9ULL;
         auto mix = [&](uint64_t hi, uint64_t lo, uint64_t far) {
             const uint64_t y = (hi & kUpper) | (lo & kLower);
-            return far ^ (y >> 1) ^ ((y & 1) ? kA : 0);
+            return far ^ (y >> 1) ^ (-(y & 1) & kA);  // branch-free, so the loops vectorise
         };
         for (int i = 0; i < kN - kM; i++) s_[i] = mix(s_[i], s_[i + 1], s_[i + kM]);
         for (int i = kN - kM; i < kN - 1; i++) s_[i] = mix(s_[i], s_[i + 1], s_[i + kM - kN]);
@@ -382,7 +382,10 @@ void grains(const Deck &d, HostDomain &h, const NucleusDraws &nd) {
             h.gid[p] = ncounter--;
         }
     }
-    for (int k = 1; k < h.nz - 1; k++)
+    // The index order of this pass does not matter, so planes run on threads: a liquid cell
+    // turns solid only when none of its neighbours is liquid, so no neighbour of a liquid cell
+    // ever changes here, and no cell reads a type that another thread writes.
+    for_planes(1, h.nz - 1, [&](int k) {
         for (int i = 1; i < h.X - 1; i++)
             for (int j = 1; j < h.Y - 1; j++) {
                 long long p = cell(h, k, i, j);
@@ -394,6 +397,7 @@ void grains(const Deck &d, HostDomain &h, const NucleusDraws &nd) {
                 }
                 if (!bordered) h.ct[p] = kSolid;
             }
+    });
 }
 
 // NucleiInit (src/CAinitialize.cpp:2213-2248, 2764-2767), rank 0: one normal draw per site in
