@@ -24,12 +24,16 @@ inline long long cell(const HostDomain &h, int k, int i, int j) {
     return (static_cast<long long>(k) * h.X + i) * h.Y + j;
 }
 
-// Runs f(k) for k = k0 .. k1-1 on up to 16 threads (f must touch only plane k's cells).
+// Runs f(k) for k = k0 .. k1-1 of h's planes on up to 16 threads, one per 256 K cells or more
+// (small domains run inline); f(k) may write only plane k's cells.
 template <typename F>
-void for_planes(int k0, int k1, F f) {
+void for_planes(const HostDomain &h, int k0, int k1, F f) {
     const int n = k1 - k0;
     if (n <= 0) return;
-    const int nt = std::min<int>(n, static_cast<int>(std::min(16u, std::max(1u, std::thread::hardware_concurrency()))));
+    const long long by_work = std::max(1LL, static_cast<long long>(h.X) * h.Y * n / (1 << 18));
+    const int nt = static_cast<int>(std::min<long long>(
+        {static_cast<long long>(n), by_work,
+         static_cast<long long>(std::min(16u, std::max(1u, std::thread::hardware_concurrency())))}));
     std::atomic<int> next{k0};
     auto work = [&] {
         for (int k; (k = next.fetch_add(1)) < k1;) f(k);
@@ -126,7 +130,7 @@ void temperature_directional(const Deck &d, HostDomain &h) {
     h.zhigh = h.nz - 1;
     h.nzA = h.nz;
     h.dU = static_cast<float>(d.R * d.deltat);
-    for_planes(0, h.nz, [&](int k) {
+    for_planes(h, 0, h.nz, [&](int k) {
         int crit = static_cast<int>(((k - 1) * d.G * d.deltax) / (d.R * d.deltat));
         for (int i = 0; i < h.X; i++)
             for (int j = 0; j < h.Y; j++) {
@@ -176,10 +180,12 @@ void temperature_spots(const Deck &d, HostDomain &h) {
 }
 
 // SubstrateInit_ConstrainedGrowth (src/CAinitialize.cpp:1460-1529), rank 0 of 1.
+// The draws happen on plane 1 only, so the other planes (walls and liquid) run on threads, and
+// the active cells to number are plane 1's.
 void substrate_directional(const Deck &d, HostDomain &h) {
     std::mt19937_64 gen(d.seed);
     std::uniform_real_distribution<double> dis(0.0, 1.0);
-    for (int k = 0; k < h.nz; k++)
+    auto plane = [&](int k) {
         for (int i = 0; i < h.X; i++)
             for (int j = 0; j < h.Y; j++) {
                 const int gx = i + h.xoff, gy = j + h.yoff;
@@ -195,8 +201,14 @@ void substrate_directional(const Deck &d, HostDomain &h) {
                     h.ct[p] = kLiquid;
                 }
             }
+    };
+    for_planes(h, 0, h.nz, [&](int k) {
+        if (k != 1) plane(k);
+    });
+    if (h.nz < 2) return;
+    plane(1);
     int next = 1;
-    for (long long p = 0; p < h.ncell; p++)
+    for (long long p = cell(h, 1, 0, 0); p < cell(h, 2, 0, 0); p++)
         if (h.ct[p] == kActive) {
             h.gid[p] = next++;
             h.substrate_active++;
@@ -342,7 +354,7 @@ struct NucleusDraws {
 };
 std::vector<long long> draws_before_planes(const HostDomain &h) {
     std::vector<long long> n(h.nz + 1, 0);
-    for_planes(1, h.nz - 1, [&](int k) {
+    for_planes(h, 1, h.nz - 1, [&](int k) {
         long long c = 0;
         for (int j = 1; j < h.Y - 1; j++)
             for (int i = 1; i < h.X - 1; i++) c += h.cts[cell(h, k, i, j)] != 0;
@@ -357,35 +369,50 @@ std::vector<long long> draws_before_planes(const HostDomain &h) {
 // below the bulk nucleation probability are potential nuclei, numbered -1, -2, ... in index
 // order; then liquid cells with no non-solid, non-wall neighbour turn solid, in index order.
 void grains(const Deck &d, HostDomain &h, const NucleusDraws &nd) {
-    size_t next = 0;
-    for (int k = 1; k < h.nz - 1 && next < nd.hits.size(); k++) {
-        if (nd.hits[next] >= nd.first[k + 1]) continue;
+    // each plane's draws are the range [first[k], first[k+1]): planes on threads
+    for_planes(h, 1, h.nz - 1, [&](int k) {
+        auto it = std::lower_bound(nd.hits.begin(), nd.hits.end(), nd.first[k]);
+        if (it == nd.hits.end() || *it >= nd.first[k + 1]) return;
         long long c = nd.first[k];
         for (int j = 1; j < h.Y - 1; j++)
             for (int i = 1; i < h.X - 1; i++) {
                 long long p = cell(h, k, i, j);
                 if (h.cts[p] == 0) continue;
-                if (next < nd.hits.size() && nd.hits[next] == c) {
+                if (it != nd.hits.end() && *it == c) {
                     h.ct[p] = kTemporaryInit;
-                    next++;
+                    ++it;
                 }
                 c++;
             }
-    }
-    int ncounter = -1;
-    const long long lo = static_cast<long long>(h.X) * h.Y, hi = static_cast<long long>(h.X) * h.Y * (h.nz - 1);
-    for (long long p = lo; p < hi; p++) {
-        if (h.ct[p] == kActive) {
-            if (p / (static_cast<long long>(h.X) * h.Y) <= h.zhigh) h.active.push_back(static_cast<uint32_t>(p));
+    });
+    // Sites are numbered -1, -2, ... in index order and the active cells listed in index order:
+    // per-plane counts first, then each plane from its offset.
+    const long long plane_cells = static_cast<long long>(h.X) * h.Y;
+    std::vector<long long> sites(h.nz + 1, 0);
+    std::vector<std::vector<uint32_t>> act(h.nz);
+    for_planes(h, 1, h.nz - 1, [&](int k) {
+        long long n = 0;
+        for (long long p = k * plane_cells; p < (k + 1) * plane_cells; p++) {
+            if (h.ct[p] == kActive) {
+                if (k <= h.zhigh) act[k].push_back(static_cast<uint32_t>(p));
+            }
+            else if (h.ct[p] == kTemporaryInit) {
+                n++;
+            }
         }
-        else if (h.ct[p] == kTemporaryInit) {
-            h.gid[p] = ncounter--;
-        }
-    }
+        sites[k + 1] = n;
+    });
+    for (int k = 1; k <= h.nz; k++) sites[k] += sites[k - 1];
+    for_planes(h, 1, h.nz - 1, [&](int k) {
+        int id = -1 - static_cast<int>(sites[k]);
+        for (long long p = k * plane_cells; p < (k + 1) * plane_cells; p++)
+            if (h.ct[p] == kTemporaryInit) h.gid[p] = id--;
+    });
+    for (int k = 1; k < h.nz - 1; k++) h.active.insert(h.active.end(), act[k].begin(), act[k].end());
     // The index order of this pass does not matter, so planes run on threads: a liquid cell
     // turns solid only when none of its neighbours is liquid, so no neighbour of a liquid cell
     // ever changes here, and no cell reads a type that another thread writes.
-    for_planes(1, h.nz - 1, [&](int k) {
+    for_planes(h, 1, h.nz - 1, [&](int k) {
         for (int i = 1; i < h.X - 1; i++)
             for (int j = 1; j < h.Y - 1; j++) {
                 long long p = cell(h, k, i, j);
