@@ -55,7 +55,7 @@ StepCtl initial_ctl(const HostDomain &hd) {
 // are temporaries).
 void upload_cells(const DomainView &d, const HostDomain &hd, hipStream_t s) {
     const size_t n = static_cast<size_t>(hd.ncell);
-    std::vector<uint8_t> ct = hd.ct;
+    std::vector<uint8_t> ct(hd.ct.begin(), hd.ct.end());
     for (uint32_t q : hd.active) ct[q] |= kSeedBit;
     uint8_t *ct_d = reinterpret_cast<uint8_t *>(d.claim_cell);
     int8_t *lid_d = reinterpret_cast<int8_t *>(ct_d + n);

@@ -518,11 +518,20 @@ HostDomain initialize(const Deck &d, bool verbose) {
     h.yoff = -1;
     h.ncell = static_cast<long long>(h.X) * h.Y * h.nz;
     if (h.ncell >= (1LL << 32) - 1) throw std::runtime_error("domain exceeds 2^32 cells");
-    h.ct.assign(h.ncell, 0);
-    h.gid.assign(h.ncell, 0);
-    h.cts.assign(h.ncell, 0);
-    h.lid.assign(h.ncell, 0);
-    h.melted.assign(h.ncell, 0);
+    // every field starts at zero: filled plane by plane on threads (first touch included)
+    h.ct.resize(h.ncell);
+    h.gid.resize(h.ncell);
+    h.cts.resize(h.ncell);
+    h.lid.resize(h.ncell);
+    h.melted.resize(h.ncell);
+    for_planes(h, 0, h.nz, [&](int k) {
+        const long long a = cell(h, k, 0, 0), b = cell(h, k + 1, 0, 0);
+        std::fill(h.ct.begin() + a, h.ct.begin() + b, 0);
+        std::fill(h.gid.begin() + a, h.gid.begin() + b, 0);
+        std::fill(h.cts.begin() + a, h.cts.begin() + b, 0);
+        std::fill(h.lid.begin() + a, h.lid.begin() + b, 0);
+        std::fill(h.melted.begin() + a, h.melted.begin() + b, 0);
+    });
     if (d.sim_type == "C")
         temperature_directional(d, h);
     else

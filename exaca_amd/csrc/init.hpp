@@ -2,11 +2,38 @@
 // serial init loops build it (src/CAinitialize.cpp:1025-2248).
 #pragma once
 #include <cstdint>
+#include <memory>
+#include <new>
+#include <utility>
 #include <vector>
 
 #include "deck.hpp"
 
 namespace exaca {
+
+// Allocator whose value-less construct leaves the element uninitialised, so that resizing a
+// whole-domain field does not zero it on one thread (the init fills the fields plane by plane on
+// threads instead).
+template <typename T>
+struct NoInitAlloc : std::allocator<T> {
+    template <typename U>
+    struct rebind {
+        using other = NoInitAlloc<U>;
+    };
+    NoInitAlloc() = default;
+    template <typename U>
+    NoInitAlloc(const NoInitAlloc<U> &) noexcept {}
+    template <typename U>
+    void construct(U *p) noexcept {
+        ::new (static_cast<void *>(p)) U;
+    }
+    template <typename U, typename... A>
+    void construct(U *p, A &&...a) {
+        ::new (static_cast<void *>(p)) U(std::forward<A>(a)...);
+    }
+};
+template <typename T>
+using Field = std::vector<T, NoInitAlloc<T>>;
 
 struct HostDomain {
     int nx = 0, ny = 0, nz = 0;   // global sizes including the wall planes
@@ -14,10 +41,10 @@ struct HostDomain {
     int xoff = -1, yoff = -1;     // MyXOffset, MyYOffset
     int nzA = 0, zlow = 0, zhigh = 0;
     long long ncell = 0;          // X*Y*nz
-    std::vector<uint8_t> ct;      // reference cell types (no seed bit)
-    std::vector<int32_t> gid, cts;
-    std::vector<int8_t> lid;
-    std::vector<uint8_t> melted;
+    Field<uint8_t> ct;            // reference cell types (no seed bit)
+    Field<int32_t> gid, cts;
+    Field<int8_t> lid;
+    Field<uint8_t> melted;
     float dU = 0;                 // UndercoolingChange shared by every cell that ever accumulates
     std::vector<int32_t> nloc, ntime;  // nucleation sites and steps (NucleiLocation/NucleationTimes)
     std::vector<float> guv;       // GrainUnitVector, 10000 x 9

@@ -18,8 +18,8 @@
 namespace exaca {
 namespace {
 
-template <typename T>
-int gathered(const HostDomain &h, const std::vector<T> &f, int k, int i, int j) {
+template <typename V>
+int gathered(const HostDomain &h, const V &f, int k, int i, int j) {
     if (k < 1 || k > h.nz - 2) return 0;
     return static_cast<int>(f[(static_cast<long long>(k) * h.X + (i + 1)) * h.Y + (j + 1)]);
 }
