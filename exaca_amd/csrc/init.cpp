@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <system_error>
 #include <thread>
 #include <cmath>
 #include <cstdio>
@@ -39,7 +40,11 @@ void for_planes(const HostDomain &h, int k0, int k1, F f) {
         for (int k; (k = next.fetch_add(1)) < k1;) f(k);
     };
     std::vector<std::thread> th;
-    for (int t = 1; t < nt; t++) th.emplace_back(work);
+    try {
+        for (int t = 1; t < nt; t++) th.emplace_back(work);
+    } catch (const std::system_error &) {
+        // fewer threads than asked for: the ones running (and this one) take every plane
+    }
     work();
     for (auto &x : th) x.join();
 }

@@ -11,6 +11,7 @@
 #include <iostream>
 #include <memory>
 #include <sstream>
+#include <system_error>
 #include <thread>
 
 #include "ca_common.hpp"
@@ -120,8 +121,14 @@ Counts write_planes(Out &o, PlaneBuffers &pb, int k0, int k1, long long cols, F 
             acc[t] = a;
         };
         std::vector<std::thread> th;
-        for (int t = 1; t < nt; t++) th.emplace_back(work, t);
+        int started = 1;
+        try {
+            for (; started < nt; started++) th.emplace_back(work, started);
+        } catch (const std::system_error &) {
+            // fewer threads than asked for: this thread formats the rest
+        }
         work(0);
+        for (int t = started; t < nt; t++) work(t);
         for (auto &x : th) x.join();
         for (int t = 0; t < nt; t++) {
             o.bytes(pb.buf[t].get(), len[t]);
