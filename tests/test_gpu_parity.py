@@ -139,3 +139,27 @@ def test_drop_in_outputs_identical(tmp_path):
         if "time" in x or "took" in x or "input file" in x:
             continue
         assert x == y
+
+
+def test_full_width_800_sample_matches_oracle(tmp_path):
+    """The 800^3 benchmark deck cut to its bottom 40 planes (804 x 804 x 42 cells: the full
+    benchmark width, and the same substrate, nuclei and times, since the random streams are drawn
+    plane by plane), 300 steps on both paths: every cell's CellType and GrainID, the active cells'
+    octahedra, and the step counters must agree."""
+    lines = (EXAMPLES / "Inp_DirSolidification_800.txt").read_text().splitlines()
+    lines = [("Domain size in z: 40" if ln.startswith("Domain size in z") else ln) for ln in lines]
+    deck = tmp_path / "cut.txt"
+    deck.write_text("\n".join(lines) + "\n")
+    osim = orc.OracleSim(deck, DATA)
+    ocycle, _ = osim.run(max_cycles=300)
+    with ea.Solver(deck) as sol:
+        gcycle, _ = sol.run(max_cycles=300)
+        q = sol.query()
+        d = osim.dims()
+        assert (gcycle, ocycle) == (300, 300)
+        assert (q["X"], q["Y"], q["nz"]) == (d["X"], d["Y"], d["nz"]) == (804, 804, 42)
+        for key in ("nn", "captures", "solidified", "frontier"):
+            assert q[key] == d[key], (key, q[key], d[key])
+        assert d["captures"] > 100000, "the sample should exercise capture at scale"
+        _compare_state(sol, osim, "800 cut to 40 planes")
+    osim.close()
