@@ -237,3 +237,25 @@ def test_spot_melt_jump_and_termination():
     ct = sim.field(orc.CELLTYPE)
     assert not np.any((ct == 5) & (lid == 0))
     assert cycle % 1000 == 0 and cycle >= cts[lid == 0].max()
+
+
+@pytest.mark.parametrize("kind", ["dirs_40_dense", "spots_small"])
+def test_counters_conserve_cells(kind, tmp_path):
+    """Cells leave the liquid only by capture or nucleation and become solid only through the
+    solidification counter, so the counters must account for every change of the CellType field
+    (a double-counted capture or a solidification counted twice fails this), part way through a
+    run and at its end."""
+    deck = (c_deck(tmp_path / "d.txt", 40, density=400, fraction=0.02) if kind == "dirs_40_dense"
+            else EXAMPLES / "Inp_SmallSpotMelt.txt")
+    sim = orc.OracleSim(deck, DATA)
+    ct0 = sim.field(orc.CELLTYPE).copy()
+    for stop in (300, 0):
+        sim.run(max_cycles=stop)
+        ct = sim.field(orc.CELLTYPE)
+        d = sim.dims()
+        n = lambda a, v: int(np.count_nonzero(a == v))
+        assert n(ct0, 5) - n(ct, 5) == d["captures"] + d["nn"], stop
+        assert n(ct, 1) - n(ct0, 1) == d["solidified"], stop
+        assert n(ct, 2) - n(ct0, 2) == d["captures"] + d["nn"] - d["solidified"], stop
+        assert np.all(ct0[ct == 5] == 5) and np.array_equal(ct0 == 0, ct == 0), stop
+    sim.close()
