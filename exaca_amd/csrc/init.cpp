@@ -129,6 +129,11 @@ inline bool in_slab(const HostDomain &h, int i, int j, int n) {
     return x >= 0 && x < h.X && y >= 0 && y < h.Y;
 }
 
+// CritTimeStep of every cell of plane k in a directional-solidification domain.
+int crit_directional(const Deck &d, int k) {
+    return static_cast<int>(((k - 1) * d.G * d.deltax) / (d.R * d.deltat));
+}
+
 // TempInit_DirSolidification (src/CAinitialize.cpp:1025-1055).
 void temperature_directional(const Deck &d, HostDomain &h) {
     h.zlow = 0;
@@ -136,7 +141,7 @@ void temperature_directional(const Deck &d, HostDomain &h) {
     h.nzA = h.nz;
     h.dU = static_cast<float>(d.R * d.deltat);
     for_planes(h, 0, h.nz, [&](int k) {
-        int crit = static_cast<int>(((k - 1) * d.G * d.deltax) / (d.R * d.deltat));
+        const int crit = crit_directional(d, k);
         for (int i = 0; i < h.X; i++)
             for (int j = 0; j < h.Y; j++) {
                 long long p = cell(h, k, i, j);
@@ -356,6 +361,7 @@ void walls_and_interface(HostDomain &h) {
 // the substrate is built.
 struct NucleusDraws {
     std::vector<long long> first, hits;
+    std::vector<uint32_t> sites;  // the cells the hits mark, in index order (filled by grains)
 };
 std::vector<long long> draws_before_planes(const HostDomain &h) {
     std::vector<long long> n(h.nz + 1, 0);
@@ -369,12 +375,22 @@ std::vector<long long> draws_before_planes(const HostDomain &h) {
     for (int k = 2; k <= h.nz; k++) n[k] += n[k - 1];
     return n;
 }
+// The same counts for a directional-solidification domain before its fields exist: every cell of
+// plane k has CritTimeStep crit_directional(k).
+std::vector<long long> draws_before_planes_directional(const Deck &d, const HostDomain &h) {
+    std::vector<long long> n(h.nz + 1, 0);
+    const long long interior = static_cast<long long>(h.X - 2) * (h.Y - 2);
+    for (int k = 1; k < h.nz - 1; k++) n[k + 1] = crit_directional(d, k) != 0 ? interior : 0;
+    for (int k = 2; k <= h.nz; k++) n[k] += n[k - 1];
+    return n;
+}
 
 // GrainInit (src/CAinitialize.cpp:1834-2206), first layer of rank 0 of 1: the cells whose draw is
 // below the bulk nucleation probability are potential nuclei, numbered -1, -2, ... in index
 // order; then liquid cells with no non-solid, non-wall neighbour turn solid, in index order.
-void grains(const Deck &d, HostDomain &h, const NucleusDraws &nd) {
+void grains(const Deck &d, HostDomain &h, NucleusDraws &nd) {
     // each plane's draws are the range [first[k], first[k+1]): planes on threads
+    std::vector<std::vector<uint32_t>> marked(h.nz);
     for_planes(h, 1, h.nz - 1, [&](int k) {
         auto it = std::lower_bound(nd.hits.begin(), nd.hits.end(), nd.first[k]);
         if (it == nd.hits.end() || *it >= nd.first[k + 1]) return;
@@ -385,11 +401,14 @@ void grains(const Deck &d, HostDomain &h, const NucleusDraws &nd) {
                 if (h.cts[p] == 0) continue;
                 if (it != nd.hits.end() && *it == c) {
                     h.ct[p] = kTemporaryInit;
+                    marked[k].push_back(static_cast<uint32_t>(p));
                     ++it;
                 }
                 c++;
             }
+        std::sort(marked[k].begin(), marked[k].end());  // draw order is (y, x), index order (x, y)
     });
+    for (int k = 1; k < h.nz - 1; k++) nd.sites.insert(nd.sites.end(), marked[k].begin(), marked[k].end());
     // Sites are numbered -1, -2, ... in index order and the active cells listed in index order:
     // per-plane counts first, then each plane from its offset.
     const long long plane_cells = static_cast<long long>(h.X) * h.Y;
@@ -433,12 +452,12 @@ void grains(const Deck &d, HostDomain &h, const NucleusDraws &nd) {
 }
 
 // NucleiInit (src/CAinitialize.cpp:2213-2248, 2764-2767), rank 0: one normal draw per site in
-// index order from a default-seeded std::default_random_engine.
-void nuclei(const Deck &d, HostDomain &h) {
+// index order from a default-seeded std::default_random_engine. The sites (the cells GrainInit
+// marked) are visited from their list, which is in index order.
+void nuclei(const Deck &d, HostDomain &h, const std::vector<uint32_t> &sites) {
     std::default_random_engine generator(static_cast<std::default_random_engine::result_type>(d.seed % 2147483646u + 1u));
     std::normal_distribution<double> distribution(d.dTN, d.dTsigma);
-    for (long long p = 0; p < h.ncell; p++) {
-        if (h.ct[p] != kTemporaryInit) continue;
+    for (const uint32_t p : sites) {
         const double und = distribution(generator);
         const int t = static_cast<int>(h.cts[p] + std::round(und / h.dU));
         h.nloc.push_back(static_cast<int32_t>(p));
@@ -523,6 +542,25 @@ HostDomain initialize(const Deck &d, bool verbose) {
     h.yoff = -1;
     h.ncell = static_cast<long long>(h.X) * h.Y * h.nz;
     if (h.ncell >= (1LL << 32) - 1) throw std::runtime_error("domain exceeds 2^32 cells");
+    // GrainInit's draws run on their own thread from the moment their count per plane is known:
+    // at once for a directional-solidification domain, after the temperature field otherwise. They
+    // read only CritTimeStep (final by then); the substrate writes only CellType and GrainID.
+    NucleusDraws nd;
+    const double bulk = d.nmax * d.deltax * d.deltax * d.deltax;
+    std::thread drawer;
+    struct JoinOnExit {
+        std::thread &t;
+        ~JoinOnExit() {
+            if (t.joinable()) t.join();
+        }
+    } join_drawer{drawer};
+    auto start_draws = [&] {
+        drawer = std::thread([&] { nd.hits = draws_below(d.seed, bulk, nd.first[std::max(h.nz - 1, 1)]); });
+    };
+    if (d.sim_type == "C") {
+        nd.first = draws_before_planes_directional(d, h);
+        start_draws();
+    }
     // every field starts at zero: filled plane by plane on threads (first touch included)
     h.ct.resize(h.ncell);
     h.gid.resize(h.ncell);
@@ -537,35 +575,28 @@ HostDomain initialize(const Deck &d, bool verbose) {
         std::fill(h.lid.begin() + a, h.lid.begin() + b, 0);
         std::fill(h.melted.begin() + a, h.melted.begin() + b, 0);
     });
-    if (d.sim_type == "C")
+    if (d.sim_type == "C") {
         temperature_directional(d, h);
-    else
+    }
+    else {
         temperature_spots(d, h);
-    // GrainInit's draws on their own thread while the substrate is built (it reads only
-    // CritTimeStep, which is final here; the substrate writes only CellType and GrainID)
-    NucleusDraws nd;
-    nd.first = draws_before_planes(h);
-    const double bulk = d.nmax * d.deltax * d.deltax * d.deltax;
-    std::thread drawer([&] { nd.hits = draws_below(d.seed, bulk, nd.first[std::max(h.nz - 1, 1)]); });
-    try {
-        h.guv = read_orientations(d.orientation_file);
-        if (d.sim_type == "C") {
-            substrate_directional(d, h);
-        }
-        else {
-            if (d.use_substrate_file)
-                substrate_file(d, h);
-            else
-                substrate_spacing(d, h);
-            walls_and_interface(h);
-        }
-    } catch (...) {
-        drawer.join();
-        throw;
+        nd.first = draws_before_planes(h);
+        start_draws();
+    }
+    h.guv = read_orientations(d.orientation_file);
+    if (d.sim_type == "C") {
+        substrate_directional(d, h);
+    }
+    else {
+        if (d.use_substrate_file)
+            substrate_file(d, h);
+        else
+            substrate_spacing(d, h);
+        walls_and_interface(h);
     }
     drawer.join();
     grains(d, h, nd);
-    nuclei(d, h);
+    nuclei(d, h, nd.sites);
     h.A = d.A * d.deltat / d.deltax;
     h.B = d.B * d.deltat / d.deltax;
     h.C = d.C * d.deltat / d.deltax;
