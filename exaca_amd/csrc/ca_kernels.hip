@@ -820,6 +820,17 @@ __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t 
                 const Coord c = decode(jc, gr);
                 const Coord cqc{c.x + nb_dx(l), c.y + nb_dy(l), c.z + nb_dz(l)};
                 const uint32_t valid = valid_neighbours(cqc, gr.X, gr.Y, gr.nzA);
+                // the grain's orientation, loaded with the neighbours' words below rather than
+                // after the mask atomics
+                const int h = won_gid[lo];
+                int o = orientation_of(h);
+                if (o < 0) {
+                    atomicOr(&ctl->error, 2);
+                    o = 0;
+                }
+                float gv[9];
+#pragma unroll
+                for (int m = 0; m < 9; m++) gv[m] = d.guv[9 * o + m];
                 // the neighbours' words: the low word is the mask (or liquid marker), the high word
                 // of a liquid neighbour its claim word
                 unsigned long long lw[26];
@@ -854,16 +865,10 @@ __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t 
 #pragma unroll
                 for (int kk = 0; kk < 26; kk++)
                     if (((need >> kk) & 1u) && q + gr.off[kk] <= jc) keep_word(d, q + gr.off[kk])[0] = serial + 1;
-                const int h = won_gid[lo];
-                int o = orientation_of(h);
-                if (o < 0) {
-                    atomicOr(&ctl->error, 2);
-                    o = 0;
-                }
                 const double xp = (c.x + d.xoff + nb_dx(l)) + 0.5;
                 const double yp = (c.y + d.yoff + nb_dy(l)) + 0.5;
                 const double zp = (c.z + nb_dz(l)) + 0.5;
-                const Octahedron oc = new_octahedron(d.guv + 9 * o, won_cx[lo], won_cy[lo], won_cz[lo], xp, yp, zp);
+                const Octahedron oc = new_octahedron(gv, won_cx[lo], won_cy[lo], won_cz[lo], xp, yp, zp);
                 Cell &cq = d.cells[q];
                 cq.gid = h;
                 cq.dl = static_cast<float>(oc.dl);
