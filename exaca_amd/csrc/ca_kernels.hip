@@ -966,8 +966,9 @@ __device__ __forceinline__ void settle_body(const DomainView &d, int p, int laye
     const uint32_t *own = d.cand + b * kCandStage;
     const uint32_t *overflow = d.list[1 - p];
     unsigned long long nsol = 0;
-    __shared__ double gtab[kGrowStage];
-    const Growth g = staged_growth(d, gtab);
+    // the growth table is read from global memory: a candidate has mostly been growing on the
+    // plateau, where the replay needs no table entry, so staging it would cost more than it saves
+    const Growth g = growth_of(d);
     auto settle = [&](uint32_t r) {
         const CellWords w = load_cell(d.cells + r);
         if ((w.ct & kTypeMask) != kActive || !(cycle > w.cts && w.lid <= layer)) return;
