@@ -62,7 +62,8 @@ _SIGNATURES = {
     "exaca_ensemble_admit": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
     "exaca_ensemble_release": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "exaca_ensemble_advance": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
-    "exaca_ensemble_poll": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
+    "exaca_ensemble_poll": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32),
+                                           ctypes.POINTER(ctypes.c_int32)]),
     "exaca_ensemble_set_blocks": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "exaca_ensemble_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     "exaca_ensemble_query": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _I64P]),
@@ -203,9 +204,17 @@ class Ensemble:
         _ok(self._lib.exaca_ensemble_advance(self._h, steps))
 
     def poll(self) -> np.ndarray:
+        """Final cycle per slot (0 while running, -1 when empty); see poll_errors for failures."""
+        return self.poll_errors()[0]
+
+    def poll_errors(self) -> tuple[np.ndarray, np.ndarray]:
+        """(final cycles, device error flags) per slot. A slot with a nonzero flag was stopped by
+        the device at the cycle given and holds no valid result."""
         out = np.zeros(self.slots, dtype=np.int32)
-        _ok(self._lib.exaca_ensemble_poll(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
-        return out
+        err = np.zeros(self.slots, dtype=np.int32)
+        p = ctypes.POINTER(ctypes.c_int32)
+        _ok(self._lib.exaca_ensemble_poll(self._h, out.ctypes.data_as(p), err.ctypes.data_as(p)))
+        return out, err
 
     def set_blocks(self, blocks: int) -> None:
         _ok(self._lib.exaca_ensemble_set_blocks(self._h, blocks))

@@ -196,6 +196,8 @@ struct DomainView {
     uint32_t *slice_claims; // entries of each workgroup's slice of the claim lists
     uint32_t *cand;         // kCandStage solidification candidates per apply workgroup
     uint32_t *cand_count;   // candidates in each region
+    uint32_t cand_cap;      // candidates staged per apply workgroup (<= kCandStage; the rest go to
+                            // the overflow list)
     uint32_t list_cap;
     int32_t nnuclei;
     const int32_t *nloc;
@@ -218,6 +220,7 @@ struct DomainView {
     int32_t autostop;       // end the domain on the device at the first check with no liquid left
     long long liquid0;      // liquid cells of the layer at the start
     int32_t *status;        // where the device reports the final cycle of an autostopped domain
+    int32_t *status_err;    // where it reports the error flags of a domain stopped by an error
 };
 
 }  // namespace exaca

@@ -736,10 +736,15 @@ __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t 
     __shared__ int32_t won_gid[kBlock];
     __shared__ float won_cx[kBlock], won_cy[kBlock], won_cz[kBlock];
     __shared__ uint32_t cstage[kCandStage];
-    __shared__ uint32_t ncstage;
+    __shared__ uint32_t ncstage, stage_end;
     StepCtl *ctl = d.ctl;
     if (ctl->done) return;
-    if (threadIdx.x == 0) ncstage = 0;
+    // stage capacity in use (kCandStage; less only when a caller narrows it)
+    const uint32_t cap = d.cand_cap < kCandStage ? d.cand_cap : kCandStage;
+    if (threadIdx.x == 0) {
+        ncstage = 0;
+        stage_end = cap;
+    }
     uint32_t *overflow = d.list[1 - p];
     uint32_t beg[kApplyGroup], upto[kApplyGroup], n = 0;
 #pragma unroll
@@ -888,7 +893,10 @@ __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t 
                 cq.lm = liquid;
             }
             // Candidates: staged in LDS (one LDS atomic per wave); a wave that finds the stage
-            // full appends to the global list directly.
+            // full appends to the global list directly. Waves reserve disjoint stage ranges in
+            // the order of their atomics, so every wave after the first that does not fit does
+            // not fit either, and every range below that wave's base is filled: stage_end (the
+            // least base of a wave that did not fit) bounds the entries copied out below.
             const uint32_t ce = __builtin_popcount(emptied);
             uint32_t ctot;
             const uint32_t cincl = wave_inclusive_scan(ce, ctot);
@@ -896,9 +904,12 @@ __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t 
                 uint32_t cbase = 0;
                 if (lane == 63) cbase = atomicAdd(&ncstage, ctot);
                 cbase = __shfl(cbase, 63, 64);
-                const bool staged = cbase + ctot <= kCandStage;
+                const bool staged = cbase + ctot <= cap;
                 if (!staged) {
-                    if (lane == 63) cbase = atomicAdd(&ctl->ncand, ctot);
+                    if (lane == 63) {
+                        atomicMin(&stage_end, cbase);
+                        cbase = atomicAdd(&ctl->ncand, ctot);
+                    }
                     cbase = __shfl(cbase, 63, 64);
                 }
                 uint32_t pos = cbase + cincl - ce;
@@ -913,8 +924,10 @@ __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t 
         }
         __syncthreads();
     }
-    // The staged candidates go to this workgroup's region.
-    const uint32_t staged = ncstage < kCandStage ? ncstage : kCandStage;
+    // The staged candidates go to this workgroup's region (the barrier orders thread 0's
+    // initialisation before these reads when the workgroup had no claims).
+    __syncthreads();
+    const uint32_t staged = ncstage < stage_end ? ncstage : stage_end;
     for (uint32_t i = threadIdx.x; i < staged; i += kBlock) d.cand[a * kCandStage + i] = cstage[i];
     if (threadIdx.x == 0) d.cand_count[a] = staged;
     put_part(d, kPartCap, a, ncap, sh);
@@ -1138,6 +1151,16 @@ __device__ __forceinline__ void close_rebuild(const DomainView &d, int p, int ac
     ctl->nactive = active + v[kPartCap] - sol;
     const int cycle = ctl->cycle[p];
     ctl->last_cycle = cycle;
+    // A domain whose kernels raised an error flag stops here: its later steps would work on a
+    // truncated list. The host learns of it at its next check (plain path) or poll (ensemble).
+    const int32_t err = ctl->error;
+    if (err) {
+        ctl->done = 1;
+        ctl->final_cycle = cycle;
+        if (d.status_err) *d.status_err = err;
+        if (d.status) *d.status = cycle;
+        return;
+    }
     if (d.autostop && cycle % 1000 == 0 &&
         d.liquid0 - static_cast<long long>(st.captures) - static_cast<long long>(st.activated) == 0) {
         ctl->done = 1;

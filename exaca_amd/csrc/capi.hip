@@ -181,6 +181,7 @@ void build_device_state(exaca_state *h) {
     d.slice_claims = h->alloc<uint32_t>(h->blocks);
     d.cand = h->alloc<uint32_t>(static_cast<size_t>(apply_blocks(h->blocks)) * kCandStage);
     d.cand_count = h->alloc<uint32_t>(apply_blocks(h->blocks));
+    d.cand_cap = cand_stage_cap();
     const size_t nwords = (n + 31) / 32;
     d.nchunks = static_cast<uint32_t>((nwords + 2047) / 2048);
     d.alarms = h->alloc<uint32_t>(kAlarmSlots * alarm_stride(d.ncell));
@@ -314,7 +315,7 @@ void device_check(exaca_state *h, int *cycle, int layer, uint64_t counts[5], int
     HIP_OK(hipMemcpyAsync(&st, h->stats, sizeof st, hipMemcpyDeviceToHost, h->stream));
     HIP_OK(hipMemcpyAsync(&err, &h->ctl->error, 4, hipMemcpyDeviceToHost, h->stream));
     HIP_OK(hipStreamSynchronize(h->stream));
-    if (err) throw DeviceError("device error flag " + std::to_string(err) + " (1: frontier overflow, 2: grain 0)");
+    if (err) throw DeviceError(exaca::device_error_text(err));
     if (counts)
         for (int k = 0; k < 5; k++) counts[k] = r.counts[k];
     // The range of active cells only widens between checks; narrow it to the cells active now.
@@ -369,7 +370,7 @@ void run_layer(exaca_state *h, int layer, int max_cycles, int verbose, int *out_
     (void)hipEventDestroy(e1);
     int32_t err = 0;
     HIP_OK(hipMemcpy(&err, &h->ctl->error, 4, hipMemcpyDeviceToHost));
-    if (err) throw DeviceError("device error flag " + std::to_string(err) + " (1: frontier overflow, 2: grain 0)");
+    if (err) throw DeviceError(exaca::device_error_text(err));
     if (out_cycle) *out_cycle = h->cycle;
     if (gpu_ms) *gpu_ms = ms;
 }

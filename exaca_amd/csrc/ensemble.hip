@@ -112,8 +112,8 @@ struct exaca_ensemble {
     Arena arena;
     exaca::DomainView *views_d = nullptr;
     std::vector<exaca::DomainView> views;
-    int32_t *status_d = nullptr;
-    std::vector<int32_t> status;
+    int32_t *status_d = nullptr;  // per slot: final cycle (slots words), then error flags (slots words)
+    std::vector<int32_t> status;  // both, as last polled
     std::vector<size_t> off, len;
     std::vector<char> used;
     std::vector<exaca::StepStats *> stats_d;
@@ -204,6 +204,7 @@ void admit(exaca_ensemble *e, const exaca_prepared *pr, int *slot_out) {
     d.slice_claims = reinterpret_cast<uint32_t *>(carve(4 * kMaxSlices));
     d.cand = reinterpret_cast<uint32_t *>(carve(4ull * exaca::apply_blocks(kMaxSlices) * exaca::kCandStage));
     d.cand_count = reinterpret_cast<uint32_t *>(carve(4 * exaca::apply_blocks(kMaxSlices)));
+    d.cand_cap = exaca::cand_stage_cap();
     d.list_cap = static_cast<uint32_t>(n);
     int32_t *nloc = reinterpret_cast<int32_t *>(carve(4 * std::max<size_t>(hd.nloc.size(), 1)));
     int32_t *ntime = reinterpret_cast<int32_t *>(carve(4 * std::max<size_t>(hd.nloc.size(), 1)));
@@ -227,6 +228,7 @@ void admit(exaca_ensemble *e, const exaca_prepared *pr, int *slot_out) {
     d.autostop = 1;
     d.liquid0 = pr->liquid0;
     d.status = e->status_d + slot;
+    d.status_err = e->status_d + e->slots + slot;
 
     hipStream_t s = e->stream;
     if (!hd.nloc.empty()) {
@@ -242,6 +244,7 @@ void admit(exaca_ensemble *e, const exaca_prepared *pr, int *slot_out) {
     HIP_OK(hipMemsetAsync(d.part, 0, 8ull * kPartRows * kMaxSlices, s));
     const int32_t zero = 0;
     HIP_OK(hipMemcpyAsync(e->status_d + slot, &zero, 4, hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemcpyAsync(e->status_d + e->slots + slot, &zero, 4, hipMemcpyHostToDevice, s));
     HIP_OK(hipMemcpyAsync(e->views_d + slot, &d, sizeof d, hipMemcpyHostToDevice, s));
     HIP_OK(hipStreamSynchronize(s));  // the host-side sources above are temporaries
     e->views[slot] = d;
@@ -249,6 +252,7 @@ void admit(exaca_ensemble *e, const exaca_prepared *pr, int *slot_out) {
     e->len[slot] = pr->bytes;
     e->used[slot] = 1;
     e->status[slot] = 0;
+    e->status[e->slots + slot] = 0;
     *slot_out = slot;
 }
 
@@ -261,6 +265,7 @@ void release(exaca_ensemble *e, int slot) {
     e->views[slot] = empty;
     e->used[slot] = 0;
     e->status[slot] = -1;
+    e->status[e->slots + slot] = 0;
 }
 
 void advance(exaca_ensemble *e, int steps) {
@@ -271,10 +276,13 @@ void advance(exaca_ensemble *e, int steps) {
 }
 
 void poll(exaca_ensemble *e) {
-    HIP_OK(hipMemcpyAsync(e->status.data(), e->status_d, 4 * e->slots, hipMemcpyDeviceToHost, e->stream));
+    HIP_OK(hipMemcpyAsync(e->status.data(), e->status_d, 8 * e->slots, hipMemcpyDeviceToHost, e->stream));
     HIP_OK(hipStreamSynchronize(e->stream));
     for (int s = 0; s < e->slots; s++)
-        if (!e->used[s]) e->status[s] = -1;
+        if (!e->used[s]) {
+            e->status[s] = -1;
+            e->status[e->slots + s] = 0;
+        }
 }
 
 unsigned long long frontier_total(exaca_ensemble *e) {
@@ -341,10 +349,11 @@ int exaca_ensemble_create(int slots, size_t arena_bytes, exaca_ensemble **out) {
         e->arena.free[0] = arena_bytes;
         HIP_OK(hipMalloc(&e->views_d, sizeof(exaca::DomainView) * slots));
         HIP_OK(hipMemset(e->views_d, 0, sizeof(exaca::DomainView) * slots));
-        HIP_OK(hipMalloc(&e->status_d, 4 * slots));
-        HIP_OK(hipMemset(e->status_d, 0, 4 * slots));
+        HIP_OK(hipMalloc(&e->status_d, 8 * slots));
+        HIP_OK(hipMemset(e->status_d, 0, 8 * slots));
         e->views.assign(slots, exaca::DomainView{});
-        e->status.assign(slots, -1);
+        e->status.assign(2 * static_cast<size_t>(slots), -1);
+        std::fill(e->status.begin() + slots, e->status.end(), 0);
         e->off.assign(slots, 0);
         e->len.assign(slots, 0);
         e->used.assign(slots, 0);
@@ -373,11 +382,12 @@ int exaca_ensemble_advance(exaca_ensemble *e, int steps) {
     });
 }
 
-int exaca_ensemble_poll(exaca_ensemble *e, int32_t *final_cycles) {
+int exaca_ensemble_poll(exaca_ensemble *e, int32_t *final_cycles, int32_t *errors) {
     return guarded([&] {
         if (!e || !final_cycles) throw std::invalid_argument("null argument");
         poll(e);
-        std::copy(e->status.begin(), e->status.end(), final_cycles);
+        std::copy(e->status.begin(), e->status.begin() + e->slots, final_cycles);
+        if (errors) std::copy(e->status.begin() + e->slots, e->status.end(), errors);
     });
 }
 

@@ -3,10 +3,12 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
 #include <stdexcept>
 #include <string>
 
 #include "../../include/exaca_ca.h"
+#include "ca_common.hpp"
 
 namespace exaca {
 
@@ -21,6 +23,29 @@ struct StateError : std::runtime_error {
 
 inline void hip_check(hipError_t e, const char *what) {
     if (e != hipSuccess) throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Device error flags (StepCtl::error) and their meaning, for the messages of the ABI calls.
+inline std::string device_error_text(int flags) {
+    std::string s = "device error flag " + std::to_string(flags) + " (";
+    const char *sep = "";
+    if (flags & 1) s += sep, s += "1: a cell list overflowed", sep = ", ";
+    if (flags & 2) s += sep, s += "2: an active cell with GrainID 0", sep = ", ";
+    if (flags & 8) s += sep, s += "8: a list-rebuild look-back did not end within its bound", sep = ", ";
+    if (flags & ~11) s += sep, s += "unknown bits";
+    return s + ")";
+}
+
+// Candidates staged in LDS per apply workgroup: kCandStage, or fewer when EXACA_CAND_STAGE
+// (1 .. kCandStage) asks for it, so that tests can drive the apply pass's overflow list on small
+// domains.
+inline uint32_t cand_stage_cap() {
+    const char *env = std::getenv("EXACA_CAND_STAGE");
+    if (!env) return kCandStage;
+    const long v = std::strtol(env, nullptr, 10);
+    if (v < 1 || v > static_cast<long>(kCandStage))
+        throw std::invalid_argument("EXACA_CAND_STAGE must be 1 .. " + std::to_string(kCandStage));
+    return static_cast<uint32_t>(v);
 }
 
 template <typename F>

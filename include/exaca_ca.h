@@ -133,8 +133,10 @@ int exaca_ensemble_release(exaca_ensemble *e, int slot);
 /* Enqueues `steps` more steps of every slot (rounded up to 50-step graph replays; asynchronous). */
 int exaca_ensemble_advance(exaca_ensemble *e, int steps);
 /* Waits for the queued steps; final_cycles[slots] receives each slot's final cycle, 0 while it
- * runs, -1 when empty. */
-int exaca_ensemble_poll(exaca_ensemble *e, int32_t *final_cycles);
+ * runs, -1 when empty. errors[slots] (may be NULL) receives each slot's device error flags (the
+ * flags of exaca_last_error's device messages; 0: none). A domain that raises one is stopped on
+ * the device at the end of that step: its final cycle is then that step and its result is void. */
+int exaca_ensemble_poll(exaca_ensemble *e, int32_t *final_cycles, int32_t *errors);
 /* Workgroups per slot for the frontier kernels (multiple of 8, 8..256). */
 int exaca_ensemble_set_blocks(exaca_ensemble *e, int blocks);
 /* Online autotuner: runs chunks_per_candidate chunks per workgroup-count candidate on the live

@@ -56,8 +56,14 @@ def test_seed_zero_plain_path_is_the_oracle_case(tmp_path):
     assert np.array_equal(ct, o.field(orc.CELLTYPE)) and np.array_equal(gid, o.field(orc.GRAINID))
 
 
-@pytest.mark.parametrize("blocks", [16, 64])
-def test_batched_ensemble_matches_each_domain_alone(tmp_path, blocks):
+@pytest.mark.parametrize("blocks,stage,tune", [(16, None, False), (64, None, False), (8, 2, False),
+                                              (64, None, True)])
+def test_batched_ensemble_matches_each_domain_alone(tmp_path, monkeypatch, blocks, stage, tune):
+    """stage: the apply pass's LDS candidate stage narrowed to that many entries (its overflow
+    list then carries almost every candidate); tune: the online autotuner picks the workgroups
+    per slot on the live ensemble after the first chunk."""
+    if stage:
+        monkeypatch.setenv("EXACA_CAND_STAGE", str(stage))
     decks = _decks(tmp_path)
     alone = [_alone(d, s) for d, s in decks]
     preps = [ea.Prepared(d, seed=s) for d, s in decks]
@@ -65,9 +71,12 @@ def test_batched_ensemble_matches_each_domain_alone(tmp_path, blocks):
         ens.set_blocks(blocks)
         slots = [ens.admit(p) for p in preps]
         assert None not in slots
-        for _ in range(10000):
+        for it in range(10000):
             ens.advance(50)
-            st = ens.poll()
+            if tune and it == 0:
+                assert ens.autotune(1) in (16, 32, 64, 128, 256)
+            st, err = ens.poll_errors()
+            assert not err.any(), err
             if all(st[s] > 0 for s in slots):
                 break
         for k, s in enumerate(slots):

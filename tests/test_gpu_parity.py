@@ -115,6 +115,26 @@ def test_full_run_matches_oracle(case, tmp_path):
         _compare_state(sol, osim, case)
 
 
+@pytest.mark.parametrize("stage", [1, 3])
+def test_candidate_overflow_path(tmp_path, monkeypatch, stage):
+    """The apply pass stages a step's solidification candidates in LDS and sends the waves that do
+    not fit to a global overflow list. With the stage narrowed to a few entries almost every
+    candidate takes the overflow path: a full run must still equal the oracle (dense nucleation:
+    many candidates per workgroup and step)."""
+    monkeypatch.setenv("EXACA_CAND_STAGE", str(stage))
+    deck = c_deck(tmp_path / "ov.txt", 40, density=400, fraction=0.02)
+    osim = orc.OracleSim(deck, DATA)
+    ocycle, _ = osim.run()
+    with ea.Solver(deck) as sol:
+        gcycle, _ = sol.run()
+        q = sol.query()
+        d = osim.dims()
+        assert gcycle == ocycle
+        for key in ("nn", "captures", "solidified", "frontier"):
+            assert q[key] == d[key], (key, q[key], d[key])
+        _compare_state(sol, osim, f"candidate stage {stage}")
+
+
 def test_drop_in_outputs_identical(tmp_path):
     """RunProgram_Reduced drop-in: vtk files byte-identical to the oracle's, log identical apart
     from the timing lines."""
