@@ -35,12 +35,15 @@ def run_sequential(requests, realtime: bool = True, log=None) -> dict:
     out = {}
     finished = {}
     gpu_ms = 0.0
+    init_s = 0.0
     for r in order:
         if realtime:
             wait = r.arrival - (time.perf_counter() - t0)
             if wait > 0:
                 time.sleep(wait)
+        ti = time.perf_counter()
         with Solver(r.deck, seed=r.seed) as sol:
+            init_s += time.perf_counter() - ti
             cycle, ms = sol.run()
             gpu_ms += ms
             out[r.index] = digest(cycle, sol.field(FIELD_CELLTYPE), sol.field(FIELD_GRAINID))
@@ -53,6 +56,7 @@ def run_sequential(requests, realtime: bool = True, log=None) -> dict:
     return {
         "wall_s": wall,
         "gpu_s": gpu_ms / 1e3,
+        "init_s": init_s,
         "digests": out,
         "solves_per_s": len(requests) / (max(finished.values()) - first),
         "p50_s": float(np.percentile(lat, 50)),
@@ -61,62 +65,83 @@ def run_sequential(requests, realtime: bool = True, log=None) -> dict:
 
 
 def run_served(requests, *, slots: int = 256, arena_bytes: int = 200 << 30, chunk: int = 50,
-               host_threads: int = 8, max_prepared: int = 48, autotune: bool = True, realtime: bool = True,
-               log=None) -> dict:
+               host_threads: int = 8, max_prepared: int = 48, autotune: bool = True, tune_at: int | None = None,
+               realtime: bool = True, max_steps: int = 5_000_000, log=None) -> dict:
     """Continuous batching. With realtime, request i becomes visible at its arrival time;
     otherwise all are visible at once (the saturated ensemble). Returns latencies (arrival to
-    completion), makespan and per-request digests."""
+    completion), makespan and per-request digests.
+
+    max_prepared bounds the requests initialised on the host and not yet admitted (their host
+    state waits in memory); admitted requests do not count, so every slot can be resident. The
+    autotuner runs once, when tune_at domains are resident (default: a quarter of min(n, slots)).
+    A domain stopped by a device error is a failed request: its digest is "error:<flags>" and its
+    index is listed under "failed". More than max_steps steps in all raise RuntimeError."""
     n = len(requests)
     order = sorted(requests, key=lambda r: r.arrival)
     pool = cf.ThreadPoolExecutor(max_workers=host_threads)
     futures = {}
+    cells = {}
     waiting = collections.deque()
     running = {}
     finished = {}
     digests = {}
+    failed = []
     admitted_at = {}
     tuned = None
+    if tune_at is None:
+        tune_at = max(1, min(n, slots) // 4)
     nxt = 0
     steps = 0
+    peak_resident = 0
     with Ensemble(slots, arena_bytes) as ens:
         t0 = time.perf_counter()
         while len(finished) < n:
             now = time.perf_counter() - t0
-            while nxt < n and (not realtime or order[nxt].arrival <= now) and len(futures) < max_prepared:
+            while nxt < n and (not realtime or order[nxt].arrival <= now) and len(waiting) < max_prepared:
                 r = order[nxt]
                 futures[r.index] = pool.submit(Prepared, r.deck, r.seed)
                 waiting.append(r)
                 nxt += 1
             while waiting and futures[waiting[0].index].done():
                 r = waiting[0]
-                slot = ens.admit(futures[r.index].result())
+                prep = futures[r.index].result()
+                slot = ens.admit(prep)
                 if slot is None:
                     break
                 waiting.popleft()
                 running[slot] = r
                 admitted_at[r.index] = time.perf_counter() - t0
+                # the device holds the state now: only the cell count is kept
+                cells[r.index] = prep.cells
+                prep.close()
+                del futures[r.index]
+            peak_resident = max(peak_resident, len(running))
             if not running:
                 time.sleep(0.0005)
                 continue
-            if autotune and tuned is None and len(running) >= min(n, slots) // 2:
+            if steps > max_steps:
+                raise RuntimeError(f"served ensemble exceeded {max_steps} steps with {len(running)} domains running")
+            if autotune and tuned is None and len(running) >= tune_at:
                 tuned = ens.autotune(1)
                 steps += 5 * chunk
             ens.advance(chunk)
             steps += chunk
-            st = ens.poll()
+            st, err = ens.poll_errors()
             done_now = time.perf_counter() - t0
-            for slot in [s for s in running if st[s] > 0]:
+            for slot in [s for s in running if st[s] > 0 or err[s] != 0]:
                 r = running.pop(slot)
-                ncell = futures[r.index].result().cells
-                ct = ens.field(slot, FIELD_CELLTYPE, ncell)
-                gid = ens.field(slot, FIELD_GRAINID, ncell)
-                digests[r.index] = digest(int(st[slot]), ct, gid)
+                if err[slot]:
+                    digests[r.index] = f"error:{int(err[slot])}"
+                    failed.append(r.index)
+                else:
+                    ncell = cells.pop(r.index)
+                    ct = ens.field(slot, FIELD_CELLTYPE, ncell)
+                    gid = ens.field(slot, FIELD_GRAINID, ncell)
+                    digests[r.index] = digest(int(st[slot]), ct, gid)
                 finished[r.index] = done_now
                 ens.release(slot)
-                fut = futures.pop(r.index)
-                fut.result().close()
                 if log:
-                    log(f"served {r.index}: cycle {st[slot]} latency {done_now - r.arrival:.3f} s")
+                    log(f"served {r.index}: cycle {st[slot]} error {int(err[slot])} latency {done_now - r.arrival:.3f} s")
         makespan = time.perf_counter() - t0
     pool.shutdown()
     lat = np.array([finished[r.index] - (r.arrival if realtime else 0.0) for r in requests])
@@ -129,5 +154,7 @@ def run_served(requests, *, slots: int = 256, arena_bytes: int = 200 << 30, chun
         "mean_s": float(lat.mean()),
         "blocks_per_slot": tuned,
         "steps": steps,
+        "peak_resident": peak_resident,
+        "failed": failed,
         "digests": digests,
     }

@@ -44,6 +44,9 @@ class FakePrepared:
 class FakeEnsemble:
     log = []
 
+    fail = set()
+    forever = set()
+
     def __init__(self, slots, arena_bytes):
         self.slots, self.free = slots, arena_bytes
         self.occ = {}
@@ -67,13 +70,17 @@ class FakeEnsemble:
 
     def advance(self, steps):
         for v in self.occ.values():
-            v[1] -= steps
+            if v[0].seed not in self.forever:
+                v[1] -= steps
 
-    def poll(self):
+    def poll_errors(self):
         st = np.full(self.slots, -1, dtype=np.int32)
+        err = np.zeros(self.slots, dtype=np.int32)
         for s, (p, left) in self.occ.items():
             st[s] = 1000 if left <= 0 else 0
-        return st
+            if p.seed in self.fail:  # the device stopped this domain with an error flag
+                st[s], err[s] = 50, 8
+        return st, err
 
     def autotune(self, k):
         return 32
@@ -110,6 +117,46 @@ def test_continuous_batching_respects_slots_and_budget(monkeypatch, slots, budge
     admits = [seed for ev, seed, _ in FakeEnsemble.log if ev == "admit"]
     assert admits == sorted(admits)  # admitted in arrival order
     assert out["p99_s"] >= out["p50_s"] >= 0
+
+
+def test_every_slot_can_be_resident(monkeypatch):
+    """The host-init cap bounds requests prepared and waiting for admission only: with room for
+    them, all slots fill even when the cap is far below the slot count."""
+    monkeypatch.setattr(serve, "Ensemble", FakeEnsemble)
+    monkeypatch.setattr(serve, "Prepared", FakePrepared)
+    monkeypatch.setattr(FakeEnsemble, "forever", {1000 + i for i in range(12)})  # first 12 never end alone
+
+    class Ens(FakeEnsemble):
+        def advance(self, steps):
+            super().advance(steps)
+            if len(self.occ) == self.slots:  # all resident: let everything finish
+                FakeEnsemble.forever = set()
+
+    monkeypatch.setattr(serve, "Ensemble", Ens)
+    out = serve.run_served(_requests(16), slots=12, arena_bytes=10_000, realtime=False, max_prepared=3,
+                           autotune=True, tune_at=12)
+    assert out["peak_resident"] == 12
+    assert out["blocks_per_slot"] == 32  # the autotuner ran once the tuning point was reached
+    assert sorted(out["digests"]) == list(range(16)) and not out["failed"]
+
+
+def test_device_error_is_a_failed_request(monkeypatch):
+    monkeypatch.setattr(serve, "Ensemble", FakeEnsemble)
+    monkeypatch.setattr(serve, "Prepared", FakePrepared)
+    monkeypatch.setattr(FakeEnsemble, "fail", {1002})
+    out = serve.run_served(_requests(5), slots=2, arena_bytes=10_000, realtime=False, autotune=False)
+    assert out["failed"] == [2]
+    assert out["digests"][2] == "error:8"
+    assert all(not out["digests"][k].startswith("error") for k in (0, 1, 3, 4))
+
+
+def test_step_bound_ends_a_stuck_loop(monkeypatch):
+    monkeypatch.setattr(serve, "Ensemble", FakeEnsemble)
+    monkeypatch.setattr(serve, "Prepared", FakePrepared)
+    monkeypatch.setattr(FakeEnsemble, "forever", {1001})
+    with pytest.raises(RuntimeError, match="exceeded 5000 steps"):
+        serve.run_served(_requests(3), slots=2, arena_bytes=10_000, realtime=False, autotune=False,
+                         max_steps=5000)
 
 
 def test_digest_identifies_results():
