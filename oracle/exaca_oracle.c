@@ -188,6 +188,7 @@ struct oracle_sim {
     int cycle;
     long long nn;
     uint64_t sum_front, captures, solidifications;
+    int mutation; /* test hook: a planted mistake in pass 2 (ORACLE_MUT_*), 0 for the restatement */
 };
 
 /* ---------------- deck parsing (src/CAinitialize.cpp:24-134) ---------------- */
@@ -955,6 +956,30 @@ int oracle_get_field(const oracle_sim *s, int which, void *dst) {
     return fail("unknown field %d", which);
 }
 
+/* ---- test hooks: hand-built states and planted mistakes (tests/ only) ---- */
+int oracle_set_field(oracle_sim *s, int which, const void *src) {
+    size_t n = (size_t)s->ndom * 4;
+    switch (which) {
+    case ORACLE_CELLTYPE: memcpy(s->ct, src, n); return 0;
+    case ORACLE_GRAINID: memcpy(s->gid, src, n); return 0;
+    case ORACLE_CRITTIMESTEP: memcpy(s->cts, src, n); return 0;
+    case ORACLE_LAYERID: memcpy(s->lid, src, n); return 0;
+    case ORACLE_UNDERCOOLING: memcpy(s->U, src, n); return 0;
+    case ORACLE_UCHANGE: memcpy(s->dU, src, n); return 0;
+    case ORACLE_MELTED: memcpy(s->melted, src, n); return 0;
+    case ORACLE_DIAGLEN: memcpy(s->DL, src, (size_t)s->nact * 4); return 0;
+    case ORACLE_DOCENTER: memcpy(s->DOC, src, (size_t)s->nact * 12); return 0;
+    case ORACLE_CRITDIAG: memcpy(s->CDL, src, (size_t)s->nact * 104); return 0;
+    }
+    return fail("field %d cannot be set", which);
+}
+int oracle_set_orientation(oracle_sim *s, int o, const float guv[9]) {
+    if (o < 0 || o >= NORIENT) return fail("orientation %d out of range", o);
+    memcpy(&s->guv[9 * o], guv, 9 * sizeof(float));
+    return 0;
+}
+void oracle_set_mutation(oracle_sim *s, int mutation) { s->mutation = mutation; }
+
 /* ======================================================================================
  * The time step
  * ====================================================================================== */
@@ -1084,7 +1109,9 @@ static void cell_capture(oracle_sim *s, int cycle, int layer) {
         }
     }
     s->sum_front += (uint64_t)nsteer;
-    for (long long q = 0; q < nsteer; q++) {
+    for (long long qq = 0; qq < nsteer; qq++) {
+        /* planted mistake ORACLE_MUT_DESCENDING: visit the steering vector from its end */
+        const long long q = s->mutation == ORACLE_MUT_DESCENDING ? nsteer - 1 - qq : qq;
         long long pa = s->steer[q];
         int rz = (int)(pa / XY);
         long long rem = pa % XY;
@@ -1099,8 +1126,10 @@ static void cell_capture(oracle_sim *s, int cycle, int layer) {
             if (mz >= s->nzA) continue;
             long long na = ((long long)mz * X + mx) * Y + my;
             long long ng = ((long long)(mz + s->zlow) * X + mx) * Y + my;
-            if (s->ct[ng] == LIQUID) lcount = 1;
-            if (!(s->DL[pa] >= s->CDL[26 * pa + l] && s->ct[ng] == LIQUID)) continue;
+            const int liquid = s->ct[ng] == LIQUID;
+            /* planted mistake ORACLE_MUT_LCOUNT: a neighbour the cell captures now does not count */
+            if (liquid && !(s->mutation == ORACLE_MUT_LCOUNT && s->DL[pa] >= s->CDL[26 * pa + l])) lcount = 1;
+            if (!(s->DL[pa] >= s->CDL[26 * pa + l] && liquid)) continue;
             s->ct[ng] = TEMP_UPDATE; /* the Serial backend's compare-exchange always succeeds here */
             int h = s->gid[pg];
             int o = orientation_of(s, h);
@@ -1118,6 +1147,8 @@ static void cell_capture(oracle_sim *s, int cycle, int layer) {
             s->DOC[3 * na + 2] = (float)c[2];
             s->ct[ng] = ACTIVE;
             s->captures++;
+            /* planted mistake ORACLE_MUT_GROW_NEW: the new cell joins this step's steering vector */
+            if (s->mutation == ORACLE_MUT_GROW_NEW) s->steer[nsteer++] = (int32_t)na;
         }
         if (lcount == 0) {
             s->ct[pg] = SOLID;

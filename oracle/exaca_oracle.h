@@ -71,6 +71,19 @@ void oracle_capture_octahedron(const float guv[9], double cxold, double cyold, d
 /* Growth increment min(0.045, max(0, V(min(210, U)))) (src/CAupdate.cpp:257-261). */
 double oracle_growth_increment(double A, double B, double C, double D, float U);
 
+/* ---- test hooks: hand-built states for the capture-order pins ----
+ * oracle_set_field overwrites a whole field (the layouts of oracle_get_field; not the nucleus
+ * arrays). oracle_set_orientation replaces one row of the orientation table (GrainUnitVector).
+ * oracle_set_mutation plants a mistake in CellCapture's pass 2 so that a test can show the pins
+ * fail it: ORACLE_MUT_DESCENDING visits the steering vector in descending index order,
+ * ORACLE_MUT_LCOUNT does not count a liquid neighbour that the cell captures as liquid,
+ * ORACLE_MUT_GROW_NEW lets a cell captured in a step grow in that step. 0 restores the
+ * restatement. */
+enum { ORACLE_MUT_NONE = 0, ORACLE_MUT_DESCENDING = 1, ORACLE_MUT_LCOUNT = 2, ORACLE_MUT_GROW_NEW = 3 };
+int oracle_set_field(oracle_sim *s, int which, const void *src);
+int oracle_set_orientation(oracle_sim *s, int o, const float guv[9]);
+void oracle_set_mutation(oracle_sim *s, int mutation);
+
 #ifdef __cplusplus
 }
 #endif

@@ -15,6 +15,8 @@ LIB = HERE / "liboracle.so"
 
 CELLTYPE, GRAINID, CRITTIMESTEP, LAYERID, UNDERCOOLING, UCHANGE, MELTED, DIAGLEN, DOCENTER, CRITDIAG, \
     NUCLEI_LOC, NUCLEI_TIME = range(12)
+# planted mistakes in CellCapture's pass 2 (oracle_set_mutation)
+MUT_NONE, MUT_DESCENDING, MUT_LCOUNT, MUT_GROW_NEW = range(4)
 
 _lib = None
 
@@ -51,6 +53,11 @@ def lib() -> ctypes.CDLL:
             ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
         L.oracle_growth_increment.argtypes = [ctypes.c_double] * 4 + [ctypes.c_float]
         L.oracle_growth_increment.restype = ctypes.c_double
+        L.oracle_set_field.argtypes = [vp, ctypes.c_int, vp]
+        L.oracle_set_field.restype = ctypes.c_int
+        L.oracle_set_orientation.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
+        L.oracle_set_orientation.restype = ctypes.c_int
+        L.oracle_set_mutation.argtypes = [vp, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -99,6 +106,23 @@ class OracleSim:
         if lib().oracle_get_field(self._h, which, out.ctypes.data) != 0:
             raise RuntimeError(lib().oracle_last_error().decode())
         return out[:cnt]
+
+    # ---- test hooks (hand-built states, planted mistakes) ----
+    def set_field(self, which: int, values: np.ndarray) -> None:
+        dt = np.float32 if which in (UNDERCOOLING, UCHANGE, DIAGLEN, DOCENTER, CRITDIAG) else np.int32
+        a = np.ascontiguousarray(values, dtype=dt)
+        if a.size != self.field(which).size:
+            raise ValueError(f"field {which}: {a.size} values for {self.field(which).size}")
+        if lib().oracle_set_field(self._h, which, a.ctypes.data) != 0:
+            raise RuntimeError(lib().oracle_last_error().decode())
+
+    def set_orientation(self, o: int, guv9) -> None:
+        g = np.ascontiguousarray(guv9, dtype=np.float32).ravel()
+        if lib().oracle_set_orientation(self._h, o, _ptr(g, ctypes.c_float)) != 0:
+            raise RuntimeError(lib().oracle_last_error().decode())
+
+    def set_mutation(self, m: int) -> None:
+        lib().oracle_set_mutation(self._h, m)
 
     def step(self, cycle: int) -> None:
         lib().oracle_step(self._h, cycle)

@@ -259,3 +259,113 @@ def test_counters_conserve_cells(kind, tmp_path):
         assert n(ct, 2) - n(ct0, 2) == d["captures"] + d["nn"] - d["solidified"], stop
         assert np.all(ct0[ct == 5] == 5) and np.array_equal(ct0 == 0, ct == 0), stop
     sim.close()
+
+
+# ---------------------------------------------------------------- capture order (Serial semantics)
+# Hand-built states on a 6^3 deck (arrays 10 x 10 x 8): every cell solid except the few placed,
+# all past their liquidus time with a constant undercooling of 100 K (growth on the 0.045 cap),
+# octahedra cell-centred with the identity orientation, so that a cell of half-diagonal d reaches
+# its face neighbours at d >= 1 and nothing else below d = 2. The expected fields are derived by
+# hand from the reference's pass 2 (src/CAupdate.cpp:245-726, Serial backend): the steering
+# vector is visited in ascending index order (:217-240); each neighbour that is liquid at the
+# visit sets LCount (:333-334) and, when reached, is captured at once by compare-exchange
+# (:336-352), so a later cell no longer sees it liquid; a cell with LCount 0 turns solid
+# (:722-725); a cell captured in this step is not in this step's steering vector (built in pass
+# 1), so it does not grow until the next step.
+WALL, SOLID, ACTIVE, LIQUID = 0, 1, 2, 5
+GROWN = np.float32(np.float64(1.0) + 0.045)  # float(1 + 0.045): one step of growth on the cap
+
+
+def _pin_state(tmp_path, cells, tag="pin"):
+    """cells: {(x, y, z): (type, grain, half-diagonal)} on a state with everything else solid.
+    Returns the oracle and a function giving the linear index of (x, y, z)."""
+    deck = c_deck(tmp_path / f"{tag}.txt", 6, density=0, fraction=0.0)
+    o = orc.OracleSim(deck, DATA)
+    d = o.dims()
+    X, Y, nz = d["X"], d["Y"], d["nz"]
+    idx = lambda x, y, z: (z * X + x) * Y + y
+    ct = o.field(orc.CELLTYPE)
+    ct[ct != WALL] = SOLID
+    gid = np.zeros_like(ct)
+    dl = np.zeros(X * Y * d["nzA"], dtype=np.float32)
+    doc = np.zeros(3 * X * Y * d["nzA"], dtype=np.float32)
+    cdl = np.zeros(26 * X * Y * d["nzA"], dtype=np.float32)
+    seed = np.abs(NB).sum(axis=1).astype(np.float32)  # identity orientation: L1 norms
+    for (x, y, z), (t, g, h) in cells.items():
+        p = idx(x, y, z)
+        assert ct[p] == SOLID, "placed cells lie inside the walls"
+        ct[p], gid[p], dl[p] = t, g, h
+        doc[3 * p:3 * p + 3] = (x - 1 + 0.5, y - 1 + 0.5, z + 0.5)  # cell centre (offset -1 in x, y)
+        cdl[26 * p:26 * p + 26] = seed
+    o.set_orientation(0, IDENTITY)  # GrainIDs 1 and 10001 both use orientation 0
+    o.set_field(orc.CELLTYPE, ct)
+    o.set_field(orc.GRAINID, gid)
+    o.set_field(orc.CRITTIMESTEP, np.full_like(ct, -100))
+    o.set_field(orc.UNDERCOOLING, np.full(ct.size, 100.0, dtype=np.float32))
+    o.set_field(orc.UCHANGE, np.zeros(ct.size, dtype=np.float32))
+    o.set_field(orc.DIAGLEN, dl)
+    o.set_field(orc.DOCENTER, doc)
+    o.set_field(orc.CRITDIAG, cdl)
+    return o, idx
+
+
+def _contest(tmp_path, mutation=orc.MUT_NONE):
+    # A (grain 1) and B (grain 10001) both reach liquid L, their common face neighbour, this step;
+    # A's index is lower (x = 3 before x = 5, same y and z)
+    o, idx = _pin_state(tmp_path, {(3, 4, 3): (ACTIVE, 1, 1.0), (4, 4, 3): (LIQUID, 0, 0.0),
+                                   (5, 4, 3): (ACTIVE, 10001, 1.0)}, tag=f"contest{mutation}")
+    o.set_mutation(mutation)
+    a, l, b = idx(3, 4, 3), idx(4, 4, 3), idx(5, 4, 3)
+    o.step(1)
+    s1 = (o.field(orc.CELLTYPE)[[a, l, b]].tolist(), int(o.field(orc.GRAINID)[l]), o.field(orc.DIAGLEN)[[a, l]])
+    o.step(2)
+    s2 = (o.field(orc.CELLTYPE)[[a, l, b]].tolist(), o.field(orc.DIAGLEN)[l])
+    return s1, s2
+
+
+def test_contested_cell_goes_to_lowest_index_and_new_cell_does_not_grow(tmp_path):
+    (ct1, gl, dl1), (ct2, dl2) = _contest(tmp_path)
+    # step 1: A captures L (grain 1); A saw L liquid at its visit and stays active; B finds no
+    # liquid neighbour and solidifies; L keeps the capture's half-diagonal (a face capture from a
+    # centred octahedron gives exactly 1) and A has grown one step
+    assert ct1 == [ACTIVE, ACTIVE, SOLID]
+    assert gl == 1
+    assert dl1[0] == GROWN and dl1[1] == np.float32(1.0)
+    # step 2: L grows for the first time; nothing liquid is left, so A and L solidify
+    assert ct2 == [SOLID, SOLID, SOLID]
+    assert dl2 == GROWN
+
+
+def test_capture_order_pins_fail_planted_mistakes(tmp_path):
+    """Each planted mistake fails the pins above: the higher index winning the contested cell, a
+    captured neighbour not counting as liquid for its capturer, a new cell growing at once."""
+    (ct1, gl, _), _ = _contest(tmp_path, orc.MUT_DESCENDING)
+    assert gl == 10001 and ct1 == [SOLID, ACTIVE, ACTIVE]
+    (ct1, gl, _), _ = _contest(tmp_path, orc.MUT_LCOUNT)
+    assert gl == 1 and ct1[0] == SOLID
+    (_, _, dl1), _ = _contest(tmp_path, orc.MUT_GROW_NEW)
+    assert dl1[1] == GROWN
+
+
+@pytest.mark.parametrize("capturer_below", [True, False])
+def test_last_liquid_neighbour_taken_lower_or_higher(tmp_path, capturer_below):
+    """Cell C (half-diagonal 0.01: reaches nothing) has one liquid neighbour L, which a capturer
+    K takes this step. K below C in index order: at C's visit L is already taken, so C turns solid
+    this step. K above C: C still sees L liquid at its visit and stays active for one more step."""
+    xc, xk = (5, 3) if capturer_below else (3, 5)
+    o, idx = _pin_state(tmp_path, {(xc, 4, 3): (ACTIVE, 1, 0.01), (4, 4, 3): (LIQUID, 0, 0.0),
+                                   (xk, 4, 3): (ACTIVE, 10001, 1.0)})
+    c, l, k = idx(xc, 4, 3), idx(4, 4, 3), idx(xk, 4, 3)
+    o.step(1)
+    ct = o.field(orc.CELLTYPE)
+    assert ct[l] == ACTIVE and o.field(orc.GRAINID)[l] == 10001
+    assert ct[c] == (SOLID if capturer_below else ACTIVE)
+    assert ct[k] == ACTIVE
+    o.step(2)
+    assert o.field(orc.CELLTYPE)[[c, l, k]].tolist() == [SOLID, SOLID, SOLID]
+    # descending visits reverse the outcome: the pin fails the planted mistake
+    o2, _ = _pin_state(tmp_path, {(xc, 4, 3): (ACTIVE, 1, 0.01), (4, 4, 3): (LIQUID, 0, 0.0),
+                                  (xk, 4, 3): (ACTIVE, 10001, 1.0)}, tag="mutated")
+    o2.set_mutation(orc.MUT_DESCENDING)
+    o2.step(1)
+    assert o2.field(orc.CELLTYPE)[c] == (ACTIVE if capturer_below else SOLID)
