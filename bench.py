@@ -2,19 +2,24 @@
 """Benchmark of the MI355X CA time step on the 800^3 directional-solidification workload.
 
 A "step" is one CA time step (Nucleation + CellCapture, one cycle of the reference's loop,
-src/runCA.cpp:290-342) over the whole 804 x 804 x 802-cell domain. After --warmup untimed steps,
-exactly --steps steps run from the device-resident state (HIP graphs, termination check every
-1000 steps as in the reference); value = active-cell updates per second, i.e. the sum over the
-timed steps of the frontier size (cells past their liquidus time, the reference's numSteer)
-divided by the device time of those steps.
+src/runCA.cpp:290-342) over the whole 804 x 804 x 802-cell domain. The run first advances untimed
+to --start-cycle (6000: the middle of the run, where the solidification front is fully developed;
+the run ends at cycle 11000), then takes --warmup untimed steps, then times exactly --steps steps
+from the device-resident state (termination check every 1000 steps as in the reference); value =
+active-cell updates per second, i.e. the sum over the timed steps of the frontier size (cells past
+their liquidus time, the reference's numSteer) divided by the device time of those steps (HIP
+events on the solver's stream).
 
 roofline: the dominant work is CellCapture (k_claim, k_apply, k_settle and k_rebuild, the
 reference's CellCapture kernel). Its algorithmic bytes are 137 B per active-cell update + 125 B per
 capture + 1 B per solidification (the reference layout's bytes: own state, the 26 critical lengths
-and the new cell's octahedron), divided by the kernels' event-timed durations over --profile-steps
-further steps. traffic is the measured HBM bytes per step of the same kernels (rocprofv3 FETCH_SIZE
-and WRITE_SIZE, calibrated on a plain copy) from the committed summary
-profiles/capture_kernels_800_hbm_traffic.md.
+and the new cell's octahedron), divided by the kernels' event-timed durations over the same cycles
+(a second solve of the deck reaches the same state and steps the window again with an event pair
+around every kernel). traffic is the measured HBM bytes per step of the same kernels over the same
+cycles (rocprofv3 FETCH_SIZE and WRITE_SIZE, calibrated on a plain copy) from the committed summary
+profiles/capture_kernels_800_hbm_traffic.md; null when that summary covers other cycles.
+With EXACA_PROFILE_REGIONS set, the two windows are bracketed by roctxProfilerResume/Pause for
+rocprofv3 --selected-regions.
 
 cpu_baseline: the oracle (the serial C restatement of the reference's algorithm) on a bounded
 sample of the same workload: the first --cpu-cycles steps of the same deck cut to its bottom
@@ -46,14 +51,19 @@ HBM_PEAK_GBPS = 8000.0
 TRAFFIC_PROFILE = REPO / "profiles" / "capture_kernels_800_hbm_traffic.md"
 
 
-def measured_traffic() -> int | None:
-    """HBM bytes per step of the CellCapture kernels from the committed rocprofv3 summary."""
+def measured_traffic() -> tuple[int | None, tuple[int, int] | None]:
+    """HBM bytes per step of the CellCapture kernels from the committed rocprofv3 summary, and the
+    cycles (first, last) its dispatches cover."""
     if not TRAFFIC_PROFILE.exists():
-        return None
+        return None, None
+    traffic = window = None
     for ln in TRAFFIC_PROFILE.read_text().splitlines():
         if ln.startswith("traffic_bytes_per_step:"):
-            return int(ln.split(":")[1])
-    return None
+            traffic = int(ln.split(":")[1])
+        if ln.startswith("traffic_cycles:"):
+            a, b = ln.split(":")[1].split("-")
+            window = (int(a), int(b))
+    return traffic, window
 
 
 def cpu_baseline(deck: Path, z: int, cycles: int) -> dict:
@@ -106,19 +116,40 @@ def ensemble_leg(n: int) -> dict:
         "sequential_p99_s": seq["p99_s"],
         "speedup_served_vs_sequential": served["solves_per_s"] / seq["solves_per_s"],
         "speedup_batched_vs_sequential": batched["solves_per_s"] / seq["solves_per_s"],
+        # the sequential loop's host initialisation runs one request at a time (threads inside a
+        # request only for large domains); without it the loop would take wall_s - init_s
+        "sequential_init_s": seq["init_s"],
+        "sequential_gpu_s": seq["gpu_s"],
+        "speedup_batched_vs_sequential_gpu_only": batched["solves_per_s"] * seq["gpu_s"] / n,
         "blocks_per_slot": batched["blocks_per_slot"],
+        "blocks_per_slot_served": served["blocks_per_slot"],
+        "peak_resident": batched["peak_resident"],
+        "peak_resident_served": served["peak_resident"],
+        "failed": sorted(set(batched["failed"]) | set(served["failed"])),
         "identical_outputs": identical,
         "arrivals": "Poisson 50 req/s (served, sequential); all at t = 0 (batched)",
     }
 
 
+def roctx():
+    """roctxProfilerResume/Pause when EXACA_PROFILE_REGIONS is set (for rocprofv3 --selected-regions:
+    only the timed windows are profiled); no-ops otherwise."""
+    if not os.environ.get("EXACA_PROFILE_REGIONS"):
+        return (lambda: None), (lambda: None)
+    import ctypes
+
+    lib = ctypes.CDLL("librocprofiler-sdk-roctx.so")
+    return (lambda: lib.roctxProfilerResume(0)), (lambda: lib.roctxProfilerPause(0))
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=6000)
-    ap.add_argument("--warmup", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--start-cycle", type=int, default=6000,
+                    help="untimed advance before the warm-up, into the run's steady front")
     ap.add_argument("--deck", default=str(REPO / "examples" / "Inp_DirSolidification_800.txt"))
-    ap.add_argument("--profile-steps", type=int, default=400)
     ap.add_argument("--cpu-z", type=int, default=40)
     ap.add_argument("--cpu-cycles", type=int, default=150)
     ap.add_argument("--no-cpu", action="store_true")
@@ -132,6 +163,7 @@ def main() -> int:
 
     import exaca_amd as ea
 
+    resume, pause = roctx()
     deck = Path(args.deck)
     t0 = time.perf_counter()
     sol = ea.Solver(deck)
@@ -144,20 +176,43 @@ def main() -> int:
         copy_ms = ea.copy_probe(copy_bytes, 10)
         copy_gbps = 2 * copy_bytes / (copy_ms / 1e3) / 1e9
 
-    if args.warmup > 0:
-        sol.run(max_cycles=args.warmup)
+    # untimed advance into the steady front, then the warm-up, then exactly --steps timed steps
+    w0 = args.start_cycle + args.warmup
+    if w0 > 0:
+        sol.run(max_cycles=w0)
     q0 = sol.query()
+    if q0["cycle"] != w0:
+        raise RuntimeError(f"the run ended at cycle {q0['cycle']} before the timed window (cycle {w0})")
     t1 = time.perf_counter()
-    cycle, gpu_ms = sol.run(max_cycles=q0["cycle"] + args.steps)
+    resume()
+    cycle, gpu_ms = sol.run(max_cycles=w0 + args.steps)
+    pause()
     wall = time.perf_counter() - t1
     q1 = sol.query()
     steps = cycle - q0["cycle"]
+    window = f"cycles {q0['cycle'] + 1}-{cycle} after an untimed advance to cycle {args.start_cycle} " \
+             f"and {args.warmup} warm-up steps"
+    print(f"[bench] timed {steps} steps ({window}) in {gpu_ms:.3f} ms device / {wall:.3f} s wall; "
+          f"{q1['frontier'] - q0['frontier']} active-cell updates, {q1['captures'] - q0['captures']} captures",
+          file=sys.stderr, flush=True)
     frontier = q1["frontier"] - q0["frontier"]
-    print(f"[bench] timed {steps} steps in {gpu_ms:.1f} ms device / {wall:.3f} s wall; "
-          f"{frontier} active-cell updates, {q1['captures'] - q0['captures']} captures", file=sys.stderr, flush=True)
     value = frontier / (gpu_ms / 1e3)
+    ncell, nuclei = sol.ncell, q1["nuclei"]
+    sol.close()
 
-    prof = sol.profile_kernels(args.profile_steps)
+    # The roofline kernels, event-timed one by one over the same window: a second solve of the same
+    # deck reaches the same state (the run is deterministic) and steps the same cycles with an event
+    # pair around every kernel. Its counters must equal the timed run's.
+    sol = ea.Solver(deck)
+    if w0 > 0:
+        sol.run(max_cycles=w0)
+    resume()
+    prof = sol.profile_kernels(steps)
+    pause()
+    sol.close()
+    for k in ("frontier", "captures", "solidified"):
+        if prof[k] != q1[k] - q0[k]:
+            raise RuntimeError(f"event-timed replay of the window differs in {k}: {prof[k]} vs {q1[k] - q0[k]}")
     kms = prof["ms"]
     nsteps = max(prof["steps"], 1)
     # CellCapture = claim + apply + settle + the rebuild of the next list (the reference's pass 1
@@ -165,9 +220,11 @@ def main() -> int:
     cap_ms = sum(v for k, v in kms.items() if k != "k_nucleate")
     bytes_alg = 137 * prof["frontier"] + 125 * prof["captures"] + 1 * prof["solidified"]
     achieved = bytes_alg / (cap_ms / 1e3) / 1e9 if cap_ms > 0 else 0.0
-    traffic = measured_traffic()
+    traffic, traffic_window = measured_traffic()
+    if traffic_window != (q0["cycle"] + 1, cycle):
+        traffic = None  # the committed counter profile covers other cycles: not comparable
     step_us = 1e3 * cap_ms / nsteps
-    sol.close()
+    measured = traffic / (step_us * 1e3) if traffic and step_us > 0 else None
 
     line = {
         "metric": METRIC,
@@ -184,11 +241,10 @@ def main() -> int:
         "data": "synthetic: the deck's own seeded substrate, nuclei and orientations",
         "config": {
             "workload": "directional solidification 800^3 (G = R = 1e6, N = 10, substrate 1/25 um^2, "
-                        "nuclei 1e13 m^-3), cycles "
-                        f"{q0['cycle'] + 1}-{cycle}",
+                        f"nuclei 1e13 m^-3), {window}",
             "deck": deck.name,
-            "cells": sol.ncell,
-            "nuclei": q1["nuclei"],
+            "cells": ncell,
+            "nuclei": nuclei,
             "parallelism": "single GPU",
         },
         "roofline": {
@@ -202,12 +258,14 @@ def main() -> int:
             "traffic_source": str(TRAFFIC_PROFILE.relative_to(REPO)) if traffic else None,
             "avg_launch_us": step_us,
             "profiled_steps": prof["steps"],
+            "profiled_window": f"cycles {q0['cycle'] + 1}-{cycle}",
             "bytes_per_update": 137,
             "bytes_per_capture": 125,
             "bytes_alg_per_step": bytes_alg / nsteps,
         },
-        # the measured HBM bytes of a step over the same event-timed duration
-        "measured_hbm_GBps": traffic / (step_us * 1e3) if traffic and step_us > 0 else None,
+        # the measured HBM bytes of a step over the same event-timed duration (never above the copy
+        # rate of the same run when both cover the same steps)
+        "measured_hbm_GBps": measured,
         "kernel_avg_us": {k: 1e3 * v / nsteps for k, v in kms.items()},
         "per_step": {k: (q1[k] - q0[k]) / max(steps, 1)
                      for k in ("frontier", "captures", "solidified", "evaluated", "masks")},
