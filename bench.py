@@ -21,10 +21,13 @@ profiles/capture_kernels_800_hbm_traffic.md; null when that summary covers other
 With EXACA_PROFILE_REGIONS set, the two windows are bracketed by roctxProfilerResume/Pause for
 rocprofv3 --selected-regions.
 
-cpu_baseline: the oracle (the serial C restatement of the reference's algorithm) on a bounded
-sample of the same workload: the first --cpu-cycles steps of the same deck cut to its bottom
---cpu-z planes (the random streams are drawn plane by plane, so the substrate, nuclei and their
-times in those planes are the 800^3 deck's own); its CellCapture time as in the reference's log.
+cpu_baseline: the oracle (the C restatement of the reference's algorithm) on the same window: it
+starts from the GPU's state at the step before the window (cell types, GrainIDs, half-diagonals,
+double octahedron centres; the oracle derives the undercooling and critical lengths as the
+reference holds them) and steps the same cycles, serially and with OpenMP on every core this
+process may use, as the reference's Serial and OpenMP backends run CellCapture; value is the
+OpenMP rate, the serial one is reported beside it, and the serial run's end state must equal the
+GPU's (serial_state_equals_gpu).
 
 ensemble: --ensemble process-map requests (256: the batched-ensemble configuration's instance
 count; directional solidification, cube sides log-uniform in
@@ -66,28 +69,85 @@ def measured_traffic() -> tuple[int | None, tuple[int, int] | None]:
     return traffic, window
 
 
-def cpu_baseline(deck: Path, z: int, cycles: int) -> dict:
-    sys.path.insert(0, str(REPO / "oracle"))
-    import oracle as orc
+def fast_leg(deck: Path, w0: int, steps: int) -> dict:
+    """The fast build (floating-point contraction allowed; judged by grain statistics, not bit
+    parity) on the same window: device time per step and its active-cell update rate."""
     import exaca_amd as ea
 
-    text = deck.read_text().splitlines()
-    text = [f"Domain size in z: {z}" if ln.startswith("Domain size in z") else ln for ln in text]
-    with tempfile.TemporaryDirectory() as d:
-        sample = Path(d) / "sample.txt"
-        sample.write_text("\n".join(text) + "\n")
-        sim = orc.OracleSim(sample, ea.DATA_DIR)
-        cycle, cap_s = sim.run(max_cycles=cycles)
-        dims = sim.dims()
+    with ea.Solver(deck, fast=True) as sol:
+        if w0 > 0:
+            sol.run(max_cycles=w0)
+        q0 = sol.query()
+        cycle, ms = sol.run(max_cycles=w0 + steps)
+        q1 = sol.query()
+    n = max(cycle - q0["cycle"], 1)
+    return {"ms_per_step": ms / n, "value": (q1["frontier"] - q0["frontier"]) / (ms / 1e3),
+            "captures_per_step": (q1["captures"] - q0["captures"]) / n,
+            "build": "libexaca_amd_fast.so: -ffp-contract=fast (fp32 divide and sqrt still correctly rounded)"}
+
+
+def host_cpu() -> tuple[str, int]:
+    """CPU model and the cores this process may use."""
+    model = "unknown"
+    try:
+        for ln in Path("/proc/cpuinfo").read_text().splitlines():
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(deck: Path, state: dict, cycle: int, steps: int, after: dict) -> dict:
+    """The oracle (the C restatement of the reference's algorithm) on the GPU line's own window:
+    it starts from the GPU's state at `cycle` (the step before the timed window) and steps the same
+    `steps` cycles, serially (the reference's Serial backend) and with OpenMP on every core this
+    process may use (the reference's OpenMP backend: atomic frontier appends and compare-exchange
+    captures). The serial run must end in the GPU's state (`after`); its CellCapture time is the
+    reference log's bucket (src/runCA.cpp:294-311)."""
+    sys.path.insert(0, str(REPO / "oracle"))
+    import numpy as np
+
+    import exaca_amd as ea
+    import oracle as orc
+
+    model, ncpu = host_cpu()
+    threads = min(ncpu, int(os.environ.get("OMP_NUM_THREADS", ncpu)))
+    legs = {}
+    parity = None
+    for name, nthreads in (("serial", 1), ("openmp", threads)):
+        t0 = time.perf_counter()
+        sim = orc.OracleSim.from_state(deck, ea.DATA_DIR, cycle, state["ct_raw"], state["gid"], state["dl"],
+                                       state["centre"], state["nloc"], state["ntime"])
+        sim.set_threads(nthreads)
+        load_s = time.perf_counter() - t0
+        end, cap_s = sim.run(max_cycles=cycle + steps)
+        d = sim.dims()
+        legs[name] = {"updates_per_s": d["frontier"] / cap_s, "capture_s": cap_s, "threads": nthreads,
+                      "steps": end - cycle, "frontier": d["frontier"], "captures": d["captures"],
+                      "import_s": load_s}
+        if name == "serial":
+            parity = bool(np.array_equal(sim.field(orc.CELLTYPE), after["ct"])
+                          and np.array_equal(sim.field(orc.GRAINID), after["gid"]))
+        print(f"[bench] cpu {name} ({nthreads} threads): {end - cycle} steps, CellCapture {cap_s:.2f} s, "
+              f"{d['frontier']} active-cell updates", file=sys.stderr, flush=True)
         sim.close()
+    omp = legs["openmp"]
     return {
-        "value": dims["frontier"] / cap_s,
+        "value": omp["updates_per_s"],
         "unit": "active-cell updates/s",
-        "cores": 1,
+        "cores": omp["threads"],
         "kind": "port",
-        "sample": f"first {cycle} steps of {deck.name} cut to its bottom {z} planes "
-                  f"({dims['X']}x{dims['Y']}x{dims['nz']} cells, same random streams); serial C oracle, "
-                  f"CellCapture time {cap_s:.2f} s for {dims['frontier']} active-cell updates",
+        "sample": f"the GPU line's own window: cycles {cycle + 1}-{cycle + steps} of {deck.name} (804x804x802), "
+                  f"started from the GPU's state at cycle {cycle}; the oracle's CellCapture run as the reference's "
+                  f"OpenMP backend on {omp['threads']} threads ({omp['capture_s']:.2f} s for {omp['frontier']} "
+                  f"active-cell updates) and serially ({legs['serial']['capture_s']:.2f} s)",
+        "serial": legs["serial"],
+        "openmp": omp,
+        "cpu_model": model,
+        "nproc": ncpu,
+        "serial_state_equals_gpu": parity,
     }
 
 
@@ -150,9 +210,8 @@ def main() -> int:
     ap.add_argument("--start-cycle", type=int, default=6000,
                     help="untimed advance before the warm-up, into the run's steady front")
     ap.add_argument("--deck", default=str(REPO / "examples" / "Inp_DirSolidification_800.txt"))
-    ap.add_argument("--cpu-z", type=int, default=40)
-    ap.add_argument("--cpu-cycles", type=int, default=150)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fast", action="store_true", help="skip the fast build's timing of the same window")
     ap.add_argument("--ensemble", type=int, default=256, help="process-map requests (0: skip)")
     ap.add_argument("--copy-probe-gib", type=int, default=4, help="size of the HBM copy calibration (0: skip)")
     args = ap.parse_args()
@@ -202,13 +261,16 @@ def main() -> int:
 
     # The roofline kernels, event-timed one by one over the same window: a second solve of the same
     # deck reaches the same state (the run is deterministic) and steps the same cycles with an event
-    # pair around every kernel. Its counters must equal the timed run's.
+    # pair around every kernel. Its counters must equal the timed run's. The CPU baseline starts
+    # from its state before the window and must end in its state after it.
     sol = ea.Solver(deck)
     if w0 > 0:
         sol.run(max_cycles=w0)
+    state = sol.front_state() if not args.no_cpu else None
     resume()
     prof = sol.profile_kernels(steps)
     pause()
+    after = {"ct": sol.field(ea.FIELD_CELLTYPE), "gid": sol.field(ea.FIELD_GRAINID)} if not args.no_cpu else None
     sol.close()
     for k in ("frontier", "captures", "solidified"):
         if prof[k] != q1[k] - q0[k]:
@@ -273,10 +335,13 @@ def main() -> int:
         "init_s": init_s,
         "wall_s": wall,
     }
+    if not args.no_fast:
+        line["fast_mode"] = fast_leg(deck, w0, steps)
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(deck, state, w0, steps, after)
+        del state, after
     if args.ensemble > 0:
         line["ensemble"] = ensemble_leg(args.ensemble)
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(deck, args.cpu_z, args.cpu_cycles)
     print(json.dumps(line), flush=True)
     return 0
 

@@ -16,6 +16,9 @@ import numpy as np
 
 PKG_DIR = Path(__file__).resolve().parent
 LIB_PATH = PKG_DIR / "libexaca_amd.so"
+# The same kernels compiled with floating-point contraction allowed: results are judged by grain
+# statistics (count, mean volume, misorientation histogram), not bit for bit.
+FAST_LIB_PATH = PKG_DIR / "libexaca_amd_fast.so"
 DATA_DIR = PKG_DIR / "data"
 
 FIELD_CELLTYPE = 0
@@ -28,6 +31,7 @@ FIELD_CELLTYPE_RAW = 6
 FIELD_NUCLEI_LOC = 7
 FIELD_NUCLEI_TIME = 8
 FIELD_MELTED = 9
+FIELD_CENTRE_F64 = 10
 
 # exported entry points and their signatures (include/exaca_ca.h)
 _I64P = ctypes.POINTER(ctypes.c_int64)
@@ -72,9 +76,10 @@ _SIGNATURES = {
     "exaca_ensemble_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "exaca_grain_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32), _I64P,
                                          ctypes.POINTER(ctypes.c_double), _I64P]),
+    "exaca_next_layer": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
 }
 
-_lib = None
+_libs = {}
 
 
 class ExacaError(RuntimeError):
@@ -85,24 +90,24 @@ class ExacaError(RuntimeError):
         self.status = status
 
 
-def library() -> ctypes.CDLL:
-    """Loads the native library once; raises if it has not been built."""
-    global _lib
-    if _lib is None:
-        if not LIB_PATH.exists():
-            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python exaca_amd/build.py`")
-        lib = ctypes.CDLL(str(LIB_PATH))
+def library(fast: bool = False) -> ctypes.CDLL:
+    """Loads the native library (or its fast build) once; raises if it has not been built."""
+    if fast not in _libs:
+        path = FAST_LIB_PATH if fast else LIB_PATH
+        if not path.exists():
+            raise RuntimeError(f"{path} is missing: build it with `python exaca_amd/build.py`")
+        lib = ctypes.CDLL(str(path))
         for name, (res, args) in _SIGNATURES.items():
-            fn = getattr(lib, name)
+            fn = getattr(lib, name)  # every entry point of include/exaca_ca.h must be exported
             fn.restype = res
             fn.argtypes = args
-        _lib = lib
-    return _lib
+        _libs[fast] = lib
+    return _libs[fast]
 
 
-def _ok(status: int) -> None:
+def _ok(status: int, lib: ctypes.CDLL | None = None) -> None:
     if status != 0:
-        raise ExacaError(status, library().exaca_last_error().decode())
+        raise ExacaError(status, (lib or library()).exaca_last_error().decode())
 
 
 def _b(s) -> bytes:
@@ -164,16 +169,19 @@ class Prepared:
 class Ensemble:
     """Many domains resident on one GPU, stepped together in batched launches."""
 
+    def _ok(self, status: int) -> None:
+        _ok(status, self._lib)
+
     def __init__(self, slots: int, arena_bytes: int):
         self._lib = library()
         h = ctypes.c_void_p()
-        _ok(self._lib.exaca_ensemble_create(slots, arena_bytes, ctypes.byref(h)))
+        self._ok(self._lib.exaca_ensemble_create(slots, arena_bytes, ctypes.byref(h)))
         self._h = h
         self.slots = slots
 
     def close(self):
         if self._h:
-            _ok(self._lib.exaca_ensemble_destroy(self._h))
+            self._ok(self._lib.exaca_ensemble_destroy(self._h))
             self._h = None
 
     def __enter__(self):
@@ -198,10 +206,10 @@ class Ensemble:
         return s.value
 
     def release(self, slot: int) -> None:
-        _ok(self._lib.exaca_ensemble_release(self._h, slot))
+        self._ok(self._lib.exaca_ensemble_release(self._h, slot))
 
     def advance(self, steps: int) -> None:
-        _ok(self._lib.exaca_ensemble_advance(self._h, steps))
+        self._ok(self._lib.exaca_ensemble_advance(self._h, steps))
 
     def poll(self) -> np.ndarray:
         """Final cycle per slot (0 while running, -1 when empty); see poll_errors for failures."""
@@ -213,36 +221,39 @@ class Ensemble:
         out = np.zeros(self.slots, dtype=np.int32)
         err = np.zeros(self.slots, dtype=np.int32)
         p = ctypes.POINTER(ctypes.c_int32)
-        _ok(self._lib.exaca_ensemble_poll(self._h, out.ctypes.data_as(p), err.ctypes.data_as(p)))
+        self._ok(self._lib.exaca_ensemble_poll(self._h, out.ctypes.data_as(p), err.ctypes.data_as(p)))
         return out, err
 
     def set_blocks(self, blocks: int) -> None:
-        _ok(self._lib.exaca_ensemble_set_blocks(self._h, blocks))
+        self._ok(self._lib.exaca_ensemble_set_blocks(self._h, blocks))
 
     def autotune(self, chunks_per_candidate: int = 2) -> int:
         c = ctypes.c_int(0)
-        _ok(self._lib.exaca_ensemble_autotune(self._h, chunks_per_candidate, ctypes.byref(c)))
+        self._ok(self._lib.exaca_ensemble_autotune(self._h, chunks_per_candidate, ctypes.byref(c)))
         return c.value
 
     def query(self, slot: int) -> dict:
         out = (ctypes.c_int64 * 8)()
-        _ok(self._lib.exaca_ensemble_query(self._h, slot, out))
+        self._ok(self._lib.exaca_ensemble_query(self._h, slot, out))
         return dict(zip(["final_cycle", "nn", "frontier", "captures", "solidified", "error", "free_bytes",
                          "steps"], list(out)))
 
     def field(self, slot: int, kind: int, ncell: int) -> np.ndarray:
         out = np.empty(ncell, dtype=np.int32)
-        _ok(self._lib.exaca_ensemble_download(self._h, slot, kind, out.ctypes.data, out.nbytes))
+        self._ok(self._lib.exaca_ensemble_download(self._h, slot, kind, out.ctypes.data, out.nbytes))
         return out
 
 
 class Solver:
-    """One CA domain resident on the GPU."""
+    """One CA domain resident on the GPU. fast=True runs the fast build (contraction allowed)."""
 
-    def __init__(self, deck, data_dir=None, verbose: bool = False, seed: int = 0):
-        self._lib = library()
+    def _ok(self, status: int) -> None:
+        _ok(status, self._lib)
+
+    def __init__(self, deck, data_dir=None, verbose: bool = False, seed: int = 0, fast: bool = False):
+        self._lib = library(fast)
         h = ctypes.c_void_p()
-        _ok(self._lib.exaca_create_seeded(_b(deck), _b(data_dir) if data_dir else None, seed, int(verbose),
+        self._ok(self._lib.exaca_create_seeded(_b(deck), _b(data_dir) if data_dir else None, seed, int(verbose),
                                           ctypes.byref(h)))
         self._h = h
         q = self.query()
@@ -251,7 +262,7 @@ class Solver:
 
     def close(self) -> None:
         if self._h:
-            _ok(self._lib.exaca_destroy(self._h))
+            self._ok(self._lib.exaca_destroy(self._h))
             self._h = None
 
     def __enter__(self):
@@ -267,10 +278,10 @@ class Solver:
             pass
 
     def nucleation(self, cycle: int, layer: int = 0) -> None:
-        _ok(self._lib.exaca_nucleation(self._h, cycle, layer))
+        self._ok(self._lib.exaca_nucleation(self._h, cycle, layer))
 
     def cell_capture(self, cycle: int, layer: int = 0) -> None:
-        _ok(self._lib.exaca_cell_capture(self._h, cycle, layer))
+        self._ok(self._lib.exaca_cell_capture(self._h, cycle, layer))
 
     def step(self, cycle: int, layer: int = 0) -> None:
         self.nucleation(cycle, layer)
@@ -280,26 +291,57 @@ class Solver:
         c = ctypes.c_int(cycle)
         counts = (ctypes.c_uint64 * 5)()
         xs = ctypes.c_int(0)
-        _ok(self._lib.exaca_check(self._h, ctypes.byref(c), layer, counts, ctypes.byref(xs), int(verbose)))
+        self._ok(self._lib.exaca_check(self._h, ctypes.byref(c), layer, counts, ctypes.byref(xs), int(verbose)))
         return c.value, list(counts), xs.value
 
-    def run(self, max_cycles: int = 0, layer: int = 0, verbose: bool = False):
-        c = ctypes.c_int(0)
-        ms = ctypes.c_double(0)
-        _ok(self._lib.exaca_run_layer(self._h, layer, max_cycles, int(verbose), ctypes.byref(c), ctypes.byref(ms)))
-        return c.value, ms.value
+    def run(self, max_cycles: int = 0, layer: int | None = None, verbose: bool = False):
+        """The layer loop of RunProgram_Reduced on the device (exaca_run_layer): with max_cycles 0
+        the current layer runs to completion and every later layer follows (exaca_next_layer in
+        between); otherwise the current layer stops after max_cycles. Returns the final cycle (of
+        the last layer run) and the device time in ms."""
+        q = self.query()
+        layer = q["layer"] if layer is None else layer
+        total = 0.0
+        while True:
+            c = ctypes.c_int(0)
+            ms = ctypes.c_double(0)
+            self._ok(self._lib.exaca_run_layer(self._h, layer, max_cycles, int(verbose), ctypes.byref(c),
+                                               ctypes.byref(ms)))
+            total += ms.value
+            if max_cycles > 0 or layer + 1 >= q["layers"]:
+                return c.value, total
+            self.next_layer(layer, verbose)
+            layer += 1
+
+    def next_layer(self, layer: int, verbose: bool = False) -> None:
+        """DomainShiftAndResize + LayerSetup after `layer` (exaca_next_layer)."""
+        self._ok(self._lib.exaca_next_layer(self._h, layer, int(verbose)))
 
     def field(self, kind: int) -> np.ndarray:
+        if kind in (FIELD_NUCLEI_LOC, FIELD_NUCLEI_TIME):
+            out = np.zeros(max(self.query()["nuclei"], 1), dtype=np.int32)
+            self._ok(self._lib.exaca_download(self._h, kind, out.ctypes.data, out.nbytes))
+            return out[:self.query()["nuclei"]]
         if kind == FIELD_CELLTYPE_RAW:
             out = np.empty(self.ncell, dtype=np.uint8)
         elif kind == FIELD_DOCENTER:
             out = np.empty(3 * self.ncell, dtype=np.float32)
+        elif kind == FIELD_CENTRE_F64:
+            out = np.empty(3 * self.ncell, dtype=np.float64)
         elif kind == FIELD_DIAGONAL:
             out = np.empty(self.ncell, dtype=np.float32)
         else:
             out = np.empty(self.ncell, dtype=np.int32)
-        _ok(self._lib.exaca_download(self._h, kind, out.ctypes.data, out.nbytes))
+        self._ok(self._lib.exaca_download(self._h, kind, out.ctypes.data, out.nbytes))
         return out
+
+    def front_state(self) -> dict:
+        """The whole state of the last step, as the reference holds it: cell types with the
+        cell-centred flag (uint8), GrainID, DiagonalLength, octahedron centres in double, and the
+        nucleation sites."""
+        return {"ct_raw": self.field(FIELD_CELLTYPE_RAW), "gid": self.field(FIELD_GRAINID),
+                "dl": self.field(FIELD_DIAGONAL), "centre": self.field(FIELD_CENTRE_F64),
+                "nloc": self.field(FIELD_NUCLEI_LOC), "ntime": self.field(FIELD_NUCLEI_TIME)}
 
     def grain_stats(self, region=None) -> dict:
         """Grain statistics of a region (XMin, XMax, YMin, YMax, ZMin, ZMax, inclusive, in the
@@ -312,23 +354,24 @@ class Solver:
         counts = (ctypes.c_int64 * 5)()
         values = (ctypes.c_double * 4)()
         hist = (ctypes.c_int64 * 64)()
-        _ok(self._lib.exaca_grain_stats(self._h, reg, counts, values, hist))
+        self._ok(self._lib.exaca_grain_stats(self._h, reg, counts, values, hist))
         return {"cells": counts[0], "melted": counts[1], "grains": counts[2], "nucleated_cells": counts[3],
                 "melted_top": counts[4], "mean_misorientation": values[0], "mean_misorientation_top": values[1],
                 "mean_grain_volume": values[2], "nucleated_fraction": values[3], "histogram": list(hist)}
 
     def query(self) -> dict:
-        out = (ctypes.c_int64 * 18)()
-        _ok(self._lib.exaca_query(self._h, out))
+        out = (ctypes.c_int64 * 20)()
+        self._ok(self._lib.exaca_query(self._h, out))
         keys = ["X", "Y", "nz", "nzA", "nx", "ny", "nuclei", "cycle", "nn", "frontier", "captures",
-                "solidified", "waiting", "list_len", "list_cap", "device_bytes", "evaluated", "masks"]
+                "solidified", "waiting", "list_len", "list_cap", "device_bytes", "evaluated", "masks", "layers",
+                "layer"]
         return dict(zip(keys, list(out)))
 
     def profile_steps(self, nsteps: int, layer: int = 0) -> dict:
         """Runs nsteps more steps with per-kernel HIP events; summed kernel times in ms."""
         ms = (ctypes.c_double * 3)()
         cnt = (ctypes.c_int64 * 5)()
-        _ok(self._lib.exaca_profile_steps(self._h, layer, nsteps, ms, cnt))
+        self._ok(self._lib.exaca_profile_steps(self._h, layer, nsteps, ms, cnt))
         return {"nucleation_ms": ms[0], "claim_ms": ms[1], "apply_ms": ms[2], "frontier": cnt[0],
                 "captures": cnt[1], "solidified": cnt[2], "launches": cnt[3], "steps": cnt[4]}
 
@@ -336,7 +379,7 @@ class Solver:
         """Runs nsteps more steps with an event pair around every kernel; summed times in ms."""
         ms = (ctypes.c_double * 5)()
         cnt = (ctypes.c_int64 * 5)()
-        _ok(self._lib.exaca_profile_kernels(self._h, layer, nsteps, ms, cnt))
+        self._ok(self._lib.exaca_profile_kernels(self._h, layer, nsteps, ms, cnt))
         names = ["k_nucleate", "k_claim", "k_apply", "k_settle", "k_rebuild"]
         return {"ms": dict(zip(names, list(ms))), "frontier": cnt[0], "captures": cnt[1], "solidified": cnt[2],
                 "steps": cnt[4]}
@@ -344,5 +387,5 @@ class Solver:
     def geometry_probe(self, rows: np.ndarray) -> np.ndarray:
         rows = np.ascontiguousarray(rows, dtype=np.float64).reshape(-1, 7)
         out = np.empty((rows.shape[0], 30), dtype=np.float64)
-        _ok(self._lib.exaca_geometry_probe(self._h, rows.ctypes.data, out.ctypes.data, rows.shape[0]))
+        self._ok(self._lib.exaca_geometry_probe(self._h, rows.ctypes.data, out.ctypes.data, rows.shape[0]))
         return out

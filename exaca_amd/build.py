@@ -1,4 +1,5 @@
-"""Builds the native library ``libexaca_amd.so`` and the ``exaca-mi355x`` executable in-tree.
+"""Builds the native library ``libexaca_amd.so`` (and its fast variant ``libexaca_amd_fast.so``) and the
+``exaca-mi355x`` executable in-tree.
 
 HIP sources are compiled for gfx950 only with hipcc; host-only C++ (deck parser, initial state,
 output writers) with g++. Floating-point contraction is off everywhere and fp32 divide/sqrt are
@@ -16,6 +17,9 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 OUT = PKG / "_build"
 LIB = PKG / "libexaca_amd.so"
+# The fast build: the same kernels with floating-point contraction allowed (fused multiply-adds),
+# judged by grain statistics instead of bit parity. Host code is shared with the exact build.
+FAST_LIB = PKG / "libexaca_amd_fast.so"
 EXE = PKG / "exaca-mi355x"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -61,6 +65,7 @@ def build(verbose: bool = False) -> Path:
         flags = [CXX_OPT.get(src, f) if f == "-O2" else f for f in CXX_FLAGS]
         _run([CXX, *flags, "-c", str(CSRC / src), "-o", str(obj)], verbose)
         objs.append(str(obj))
+    host_objs = list(objs)
     for src in HIP_SOURCES:
         obj = OUT / (src + ".o")
         _run([HIPCC, *HIP_FLAGS, "-c", str(CSRC / src), "-o", str(obj)], verbose)
@@ -68,6 +73,17 @@ def build(verbose: bool = False) -> Path:
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-o", str(tmp), *objs, "-lpthread", "-ldl"], verbose)
     os.replace(tmp, LIB)
+    # fast build: contraction allowed in the kernels (fp32 divide and sqrt stay correctly rounded)
+    fast_flags = ["-ffp-contract=fast" if f == "-ffp-contract=off" else f for f in HIP_FLAGS]
+    (OUT / "fast").mkdir(exist_ok=True)
+    fobjs = list(host_objs)
+    for src in HIP_SOURCES:
+        obj = OUT / "fast" / (src + ".o")
+        _run([HIPCC, *fast_flags, "-c", str(CSRC / src), "-o", str(obj)], verbose)
+        fobjs.append(str(obj))
+    tmp = FAST_LIB.with_suffix(".so.tmp")
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-o", str(tmp), *fobjs, "-lpthread", "-ldl"], verbose)
+    os.replace(tmp, FAST_LIB)
     _run([CXX, "-O2", "-std=c++17", "-o", str(EXE), str(CSRC / "main.cpp"), f"-L{PKG}", "-lexaca_amd",
           "-Wl,-rpath,$ORIGIN"], verbose)
     return LIB

@@ -63,6 +63,7 @@ constexpr size_t alarm_stride(long long ncell) {
     return ((static_cast<size_t>(ncell) + 31) / 32 + 7) & ~static_cast<size_t>(7);
 }
 constexpr int kMaxJumps = 4096;
+constexpr int kMaxLayers = 256;
 
 // The 26 neighbour offsets in the reference order: 0-8 at dy = -1, 9-16 in the y plane, 17-25 at
 // dy = +1 (src/CAinitialize.cpp:595-674). The order fixes the seed-table column of each neighbour.
@@ -116,6 +117,11 @@ struct StepCtl {
     unsigned long long nonwaiting;  // of those, past their liquidus time (the reference's numSteer)
     int32_t jump_from[kMaxJumps];  // cycle J before the jump; cycles J+1 .. T-1 were skipped
     int32_t jump_to[kMaxJumps];    // T, the first cycle executed after the jump
+    int32_t layer;                 // the layer being solidified; every layer's cycles start at 1
+    int32_t layer_end[kMaxLayers];   // last cycle of each finished layer
+    int32_t layer_jump0[kMaxLayers]; // index of each layer's first cycle jump
+    int32_t layer_zlo[kMaxLayers];   // each layer's active window [ZBound_Low, ZBound_High]: only
+    int32_t layer_zhi[kMaxLayers];   // cells inside it accumulate undercooling in that layer
 };
 
 // Per-run counters kept on the device.
@@ -163,6 +169,7 @@ enum FieldKind : int {
     kFieldCritTimeStep = 4,
     kFieldLayerId = 5,
     kFieldCellTypeRaw = 6,
+    kFieldCentreF64 = 10,
 };
 
 // Counters of the current step, one slot per workgroup (part[row * nparts + workgroup]): every
@@ -184,7 +191,9 @@ enum PartRow : int {
 
 // Everything a kernel needs to address one CA domain.
 struct DomainView {
-    int X, Y, nz, nzA;      // MyXSlices, MyYSlices, nz, nzActive (ZBound_Low = 0 on one GPU)
+    int X, Y, nz, nzA;      // MyXSlices, MyYSlices, nz, and the top of the active window: cells of
+                            // z < nzA (ZBound_High + 1, so nzActive while ZBound_Low = 0)
+    int zlo;                // ZBound_Low, the window's bottom plane (0 in the first layer)
     int xoff, yoff;         // MyXOffset, MyYOffset (-1 for one rank)
     long long ncell;        // X*Y*nz
     Cell *cells;

@@ -34,12 +34,13 @@ __device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b 
 
 __device__ __forceinline__ int orientation_of(int gid) { return (abs(gid) - 1) % kNumOrientations; }
 
-// Steps executed in (t, cycle]: the reference adds the undercooling change once per executed step
-// while a cell is liquid or active past its liquidus time (src/CAupdate.cpp:235-237); cycle jumps
-// (src/CAupdate.cpp:812-814) skip steps J+1 .. T-1.
-__device__ __forceinline__ int executed_since(int t, int cycle, const StepCtl *ctl, int njumps) {
+// Steps executed in (t, cycle] of one layer whose cycle jumps are entries j0 .. j1-1: the reference
+// adds the undercooling change once per executed step while a cell is liquid or active past its
+// liquidus time (src/CAupdate.cpp:235-237); cycle jumps (src/CAupdate.cpp:812-814) skip steps
+// J+1 .. T-1.
+__device__ __forceinline__ int executed_in(int t, int cycle, const StepCtl *ctl, int j0, int j1) {
     int n = cycle - t;
-    for (int q = 0; q < njumps; q++) {
+    for (int q = j0; q < j1; q++) {
         const int lo = max(ctl->jump_from[q] + 1, t + 1);
         const int hi = min(ctl->jump_to[q] - 1, cycle);
         if (hi >= lo) n -= hi - lo + 1;
@@ -339,10 +340,26 @@ __device__ __forceinline__ bool is_unclaimed_liquid(unsigned long long lm) {
 // this fixed recurrence, so it is applied when the cell is next looked at, one float rounding per
 // step exactly as the reference stores it. ----
 
-// k of the step at `cycle` (0 while the cell waits for its liquidus time).
-__device__ __forceinline__ int growth_index(const DomainView &d, int cts, int cycle) {
-    const int k = executed_since(cts, cycle, d.ctl, d.ctl->njumps);
-    return k < 0 ? 0 : k;
+// k of the step at `cycle` of the current layer for cell q: the steps executed past its liquidus
+// time in every layer from its own LayerID on in which it lay inside that layer's active window
+// (pass 1 visits the window only and accumulates only while LayerID <= layer, and the undercooling
+// is never reset between layers; src/CAupdate.cpp:217-237, src/runCA.cpp:283-418). 0 while the cell
+// waits for its liquidus time or its layer.
+__device__ __forceinline__ int growth_index(const DomainView &d, int cts, int lid, uint32_t q, int cycle) {
+    const StepCtl *c = d.ctl;
+    const int L = c->layer;
+    if (lid > L) return 0;
+    int k = executed_in(cts, cycle, c, c->layer_jump0[L], c->njumps);
+    k = k < 0 ? 0 : k;
+    if (L > 0) {
+        const int z = static_cast<int>(q / (static_cast<uint32_t>(d.X) * static_cast<uint32_t>(d.Y)));
+        for (int l = lid < 0 ? 0 : lid; l < L; l++) {
+            if (z < c->layer_zlo[l] || z > c->layer_zhi[l]) continue;
+            const int e = executed_in(cts, c->layer_end[l], c, c->layer_jump0[l], c->layer_jump0[l + 1]);
+            k += e < 0 ? 0 : e;
+        }
+    }
+    return k;
 }
 // The growth increments: grow[k] below the plateau index ksat, the plateau value from there on.
 // tab is the table or a copy of its first ksat entries in LDS (the capture pass stages one, as its
@@ -429,10 +446,11 @@ __device__ __forceinline__ void put_parts4(const DomainView &d, const int (&rows
 
 // Growth index that an active cell's dl includes, at a step of growth index k. A cell not yet
 // looked at as an active cell (mask not yet known, or built by its capture: kFresh) holds its
-// initial dl: it grows from the step it is first looked at on, so its base is k - 1 (0 while it
-// waits for its liquidus time); otherwise the base is in win.
+// initial dl: it grows from the step it is first looked at on, so its base is k - 1 (k while it
+// waits for its liquidus time or its layer: k then holds only earlier layers' steps, 0 in a
+// single-layer run); otherwise the base is in win.
 __device__ __forceinline__ int growth_base(const CellWords &w, int k, bool waiting) {
-    if ((w.lm & kLmUnknown) || (w.flags & kFresh)) return waiting ? 0 : k - 1;
+    if ((w.lm & kLmUnknown) || (w.flags & kFresh)) return waiting ? k : k - 1;
     return static_cast<int>(w.win);
 }
 // Cell r of known mask is past its liquidus time and has no liquid neighbour left: at its visit
@@ -522,7 +540,7 @@ __device__ __forceinline__ void nucleate_body(const DomainView &d, int p, int la
             const CellWords w = load_cell(d.cells + r);
             if ((w.ct & kTypeMask) != kActive || !(cycle > w.cts && w.lid <= layer)) continue;
             if (!(w.flags & kCounted)) n.entered++;
-            solidify(d, growth_of(d), r, w, growth_index(d, w.cts, cycle));
+            solidify(d, growth_of(d), r, w, growth_index(d, w.cts, w.lid, r, cycle));
             d.cells[r].flags = (w.flags & ~kFresh) | kCounted;
             n.nsol++;
         }
@@ -652,7 +670,7 @@ __device__ __forceinline__ void claim_body(const DomainView &d, int p, int layer
             const CellWords w = wc;
             if ((w.ct & kTypeMask) == kActive) {
                 const bool waiting = !(cycle > w.cts && w.lid <= layer);
-                const int k = growth_index(d, w.cts, cycle);
+                const int k = growth_index(d, w.cts, w.lid, j, cycle);
                 uint32_t liquid;
                 const int kdl = growth_base(w, k, waiting);
                 if (w.lm & kLmUnknown) {
@@ -989,7 +1007,7 @@ __device__ __forceinline__ void settle_body(const DomainView &d, int p, int laye
             set_alarm(d, serial + 1, r);
         }
         else {
-            solidify(d, g, r, w, growth_index(d, w.cts, cycle));
+            solidify(d, g, r, w, growth_index(d, w.cts, w.lid, r, cycle));
             nsol++;
         }
     };
@@ -1288,17 +1306,90 @@ __device__ __forceinline__ void rebuild_body(const DomainView &d, int p, int acc
     }
 }
 
-// Every active cell due at the next step (after a cycle jump, whose skipped steps the calendar did
-// not count).
+// Every active cell of the window due at the next step (after a cycle jump, whose skipped steps the
+// calendar did not count). Active cells above the window (substrate cells next to a later layer's
+// melt pool) are never stepped, as the reference's pass 1 never visits them.
 __device__ __forceinline__ void rewake_body(const DomainView &d, int p_next, uint32_t b, uint32_t nb) {
     StepCtl *ctl = d.ctl;
     if (b == 0 && threadIdx.x == 0) ctl->rebuild_seq = ctl->rebuild_seq + 1;
     const uint32_t lo = ctl->elo, hi = ctl->ehi;
     if (lo > hi) return;
+    const unsigned long long end = static_cast<unsigned long long>(d.X) * d.Y * d.nzA;
     const uint32_t serial = ctl->serial[p_next];
-    for (unsigned long long q = lo + static_cast<unsigned long long>(b) * kBlock + threadIdx.x; q <= hi && q < static_cast<unsigned long long>(d.ncell);
+    for (unsigned long long q = lo + static_cast<unsigned long long>(b) * kBlock + threadIdx.x; q <= hi && q < end;
          q += static_cast<unsigned long long>(nb) * kBlock)
         if ((d.cells[q].ct & kTypeMask) == kActive) set_alarm(d, serial, static_cast<uint32_t>(q));
+}
+
+// The next layer's window (src/CAinitialize.cpp:2772-2832): the plane of the lowest active cell and
+// the top plane of the next layer's cells. out[0] = least z (atomicMin), out[1] = greatest z.
+__global__ __launch_bounds__(kBlock) void k_layer_bounds(DomainView d, int next_layer, int *out) {
+    int zmin = 0x7FFFFFFF, zmax = -1;
+    const Grid gr = grid_of(d);
+    for (long long q = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x; q < d.ncell;
+         q += static_cast<long long>(gridDim.x) * kBlock) {
+        const uint32_t tail = reinterpret_cast<const uint32_t *>(d.cells + q)[7];
+        const int z = static_cast<int>(gr.xy.div(static_cast<uint32_t>(q)));
+        if (((tail >> 8) & kTypeMask) == kActive) zmin = z < zmin ? z : zmin;
+        if (static_cast<int8_t>(tail & 0xFFu) == next_layer) zmax = z > zmax ? z : zmax;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const int a = __shfl_xor(zmin, o, 64), b = __shfl_xor(zmax, o, 64);
+        zmin = a < zmin ? a : zmin;
+        zmax = b > zmax ? b : zmax;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMin(&out[0], zmin);
+        atomicMax(&out[1], zmax);
+    }
+}
+
+// LayerSetup (src/CAinitialize.cpp:2835-2982) on the resident state: every active cell of the new
+// window [zlo, nzA) gets a cell-centred octahedron (half-diagonal 0.01, centre at the cell centre,
+// critical lengths from its orientation) and an unknown liquid mask (its neighbours changed with the
+// layer), is counted afresh, and is due at the first step of the layer (the step of parity p_next).
+// The calendar was cleared before; the list is rebuilt after. ctl->nactive and the range of active
+// cells (ctl->elo, ctl->ehi, reset by the host) are accumulated here.
+__global__ __launch_bounds__(kBlock) void k_layer_setup(DomainView d, int p_next) {
+    __shared__ unsigned long long sh[kBlock / 64];
+    StepCtl *ctl = d.ctl;
+    const uint32_t serial = ctl->serial[p_next];
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->rebuild_seq = ctl->rebuild_seq + 1;
+    const Grid gr = grid_of(d);
+    const long long begin = static_cast<long long>(d.X) * d.Y * d.zlo;
+    const long long end = static_cast<long long>(d.X) * d.Y * d.nzA;
+    unsigned long long n = 0;
+    uint32_t qlo = 0xFFFFFFFFu, qhi = 0;
+    for (long long q = begin + static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x; q < end;
+         q += static_cast<long long>(gridDim.x) * kBlock) {
+        Cell &c = d.cells[q];
+        if ((c.ct & kTypeMask) != kActive) continue;
+        const Coord p = decode(static_cast<uint32_t>(q), gr);
+        c.lm = kLmUnknown;
+        c.dl = 0.01f;
+        c.thr = 0.0f;
+        c.win = 0;
+        c.flags = 0;
+        c.ct = kActive | kSeedBit;
+        d.centre[q] = Centre{(p.x + d.xoff) + 0.5, (p.y + d.yoff) + 0.5, p.z + 0.5, 0.0};
+        set_alarm(d, serial, static_cast<uint32_t>(q));
+        n++;
+        qlo = static_cast<uint32_t>(q) < qlo ? static_cast<uint32_t>(q) : qlo;
+        qhi = static_cast<uint32_t>(q) > qhi ? static_cast<uint32_t>(q) : qhi;
+    }
+    const unsigned long long t = block_sum(n, sh);
+    if (threadIdx.x == 0 && t) atomicAdd(&ctl->nactive, t);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t a = __shfl_xor(qlo, o, 64), b = __shfl_xor(qhi, o, 64);
+        qlo = a < qlo ? a : qlo;
+        qhi = b > qhi ? b : qhi;
+    }
+    if ((threadIdx.x & 63) == 0 && qlo <= qhi) {
+        atomicMin(&ctl->elo, qlo);
+        atomicMax(&ctl->ehi, qhi);
+    }
 }
 
 // Plain launches: one domain, passed by value.
@@ -1349,11 +1440,13 @@ __global__ __launch_bounds__(kBlock) void k_check(DomainView d, int cycle, int l
     __shared__ unsigned long long shmin[kBlock / 64];
     unsigned long long c[5] = {0, 0, 0, 0, 0};
     unsigned long long next = ~0ull, amin = ~0ull, amax = 0;
+    // the active window [ZBound_Low, ZBound_High] (src/CAupdate.cpp:792-808 reduces over it)
+    const long long active_begin = static_cast<long long>(d.X) * d.Y * d.zlo;
     const long long active_end = static_cast<long long>(d.X) * d.Y * d.nzA;
     for (long long q = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x; q < d.ncell;
          q += static_cast<long long>(gridDim.x) * kBlock) {
         const int t = d.cells[q].ct & kTypeMask;
-        if (t == kActive) {
+        if (t == kActive && q >= active_begin && q < active_end) {
             amin = static_cast<unsigned long long>(q) < amin ? static_cast<unsigned long long>(q) : amin;
             amax = static_cast<unsigned long long>(q) > amax ? static_cast<unsigned long long>(q) : amax;
         }
@@ -1365,7 +1458,7 @@ __global__ __launch_bounds__(kBlock) void k_check(DomainView d, int cycle, int l
                 else
                     c[1]++;
                 const unsigned long long v = static_cast<unsigned long long>(static_cast<long long>(crit));
-                if (q < active_end && v < next) next = v;
+                if (q >= active_begin && q < active_end && v < next) next = v;
             }
             else if (t == kActive)
                 c[2]++;
@@ -1469,6 +1562,49 @@ __global__ __launch_bounds__(kBlock) void k_grain_stats(DomainView d, const uint
         if (bins[o]) atomicAdd(&obins[o], bins[o]);
 }
 
+// The initial state of a directional-solidification domain built on the device
+// (TempInit_DirSolidification, SubstrateInit_ConstrainedGrowth, GrainInit and NucleiInit of
+// src/CAinitialize.cpp:1025-1055, 1460-1529, 1834-2248): every cell is a wall (the ghost ring and
+// the bottom and top planes) or liquid, with its plane's CritTimeStep, LayerID 0 and the Melted
+// flag; then the substrate's active cells (GrainIDs 1, 2, ... in list order, cell-centred octahedra
+// of half-diagonal 0.01) and the nucleation sites (GrainIDs -1, -2, ... in list order) from the
+// host's draws. No cell of such a domain is solid, so GrainInit's last pass (liquid cells without a
+// non-solid, non-wall neighbour turn solid) changes nothing when the interior is at least two
+// cells wide in x and y.
+__global__ __launch_bounds__(kBlock) void k_init_directional(DomainView d, const int32_t *plane_cts, int nx, int ny,
+                                                             uint8_t *melted) {
+    const Grid gr = grid_of(d);
+    for (long long q = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x; q < d.ncell;
+         q += static_cast<long long>(gridDim.x) * kBlock) {
+        const Coord c = decode(static_cast<uint32_t>(q), gr);
+        const int gx = c.x + d.xoff, gy = c.y + d.yoff;
+        const bool wall = gx == -1 || gx == nx || gy == -1 || gy == ny || c.z == 0 || c.z == d.nz - 1;
+        const uint8_t ct = wall ? kWall : kLiquid;
+        uint4 *r = reinterpret_cast<uint4 *>(d.cells + q);
+        r[0] = make_uint4(wall ? kMaskUnknown : 0xFFFFFFFFu, wall ? 0u : kClaimNone, 0u, 0u);
+        r[1] = make_uint4(static_cast<uint32_t>(plane_cts[c.z]), 0u, 0u, static_cast<uint32_t>(ct) << 8);
+        d.centre[q] = Centre{0.0, 0.0, 0.0, 0.0};
+        if (melted) melted[q] = wall ? 0 : 1;
+    }
+}
+__global__ __launch_bounds__(kBlock) void k_init_substrate(DomainView d, const uint32_t *cells, uint32_t n) {
+    const Grid gr = grid_of(d);
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const uint32_t q = cells[i];
+        const Coord c = decode(q, gr);
+        Cell &r = d.cells[q];
+        r.lm = kLmUnknown;
+        r.dl = 0.01f;
+        r.gid = static_cast<int32_t>(i + 1);
+        r.ct = kActive | kSeedBit;
+        d.centre[q] = Centre{(c.x + d.xoff) + 0.5, (c.y + d.yoff) + 0.5, c.z + 0.5, 0.0};
+    }
+}
+__global__ __launch_bounds__(kBlock) void k_init_sites(DomainView d, const int32_t *sites, uint32_t n) {
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+        d.cells[sites[i]].gid = -static_cast<int32_t>(i + 1);
+}
+
 // Initial cell records from the host's compact arrays: no octahedron, no claim, liquid mask not
 // yet known; the centres start at zero.
 __global__ __launch_bounds__(kBlock) void k_pack(DomainView d, const uint8_t *ct, const int8_t *lid,
@@ -1497,7 +1633,7 @@ __global__ __launch_bounds__(kBlock) void k_extract(DomainView d, int field, voi
         case kFieldDiagonal: {  // float, with the growth of the steps since the cell was last looked at
             float dl = c.dl;
             if ((c.ct & kTypeMask) == kActive && !(c.lm & kLmUnknown) && !(c.flags & kFresh))
-                dl = replay_growth(growth_of(d), dl, static_cast<int>(c.win), growth_index(d, c.cts, d.ctl->last_cycle));
+                dl = replay_growth(growth_of(d), dl, static_cast<int>(c.win), growth_index(d, c.cts, c.lid, static_cast<uint32_t>(q), d.ctl->last_cycle));
             static_cast<float *>(out)[q] = dl;
             break;
         }
@@ -1509,6 +1645,14 @@ __global__ __launch_bounds__(kBlock) void k_extract(DomainView d, int field, voi
             o[0] = static_cast<float>(e.x);
             o[1] = static_cast<float>(e.y);
             o[2] = static_cast<float>(e.z);
+            break;
+        }
+        case kFieldCentreF64: {  // double x, y, z, as computed at capture
+            const Centre &e = d.centre[q];
+            double *o = static_cast<double *>(out) + 3 * q;
+            o[0] = e.x;
+            o[1] = e.y;
+            o[2] = e.z;
             break;
         }
         default: break;
@@ -1531,6 +1675,22 @@ __global__ __launch_bounds__(kBlock) void k_geometry_probe(const float *guv, con
     b[3] = static_cast<float>(o.dl);
     const Faces f = faces_of(g);
     for (int l = 0; l < 26; l++) b[4 + l] = critical_length(f, a[4], a[5], a[6], o.cx, o.cy, o.cz, l);
+}
+
+// Unique GrainIDs of a region: the flags k_grain_stats set, counted into *out.
+__global__ __launch_bounds__(kBlock) void k_count_flags(const uint32_t *gflag, uint32_t n, unsigned long long *out) {
+    __shared__ unsigned long long sh[kBlock / 64];
+    unsigned long long c = 0;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) c += gflag[i] != 0;
+    const unsigned long long t = block_sum(c, sh);
+    if (threadIdx.x == 0 && t) atomicAdd(out, t);
+}
+
+// Fill of n 32-bit words (a kernel rather than a memset node, so that a caller may capture it).
+__global__ __launch_bounds__(kBlock) void k_fill(uint32_t *p, size_t n, uint32_t v) {
+    for (size_t i = static_cast<size_t>(blockIdx.x) * kBlock + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * kBlock)
+        p[i] = v;
 }
 
 __global__ __launch_bounds__(kBlock) void k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n) {
@@ -1579,6 +1739,15 @@ void launch_apply(const DomainView &d, int parity, int layer, int blocks, hipStr
     k_apply<<<apply_blocks(blocks), kBlock, 0, s>>>(d, parity, blocks);
     k_settle<<<settle_blocks(blocks), kBlock, 0, s>>>(d, parity, layer);
     k_rebuild<<<cb, kBlock, 0, s>>>(d, parity, 1, blocks, grid_for(d.nnuclei, 1024));
+}
+void launch_layer_bounds(const DomainView &d, int next_layer, int *out, hipStream_t s) {
+    k_layer_bounds<<<grid_for(d.ncell, 4096), kBlock, 0, s>>>(d, next_layer, out);
+}
+void launch_layer_setup(const DomainView &d, int parity_next, int blocks, hipStream_t s) {
+    const int cb = rebuild_blocks(d, blocks);
+    const long long n = static_cast<long long>(d.X) * d.Y * (d.nzA - d.zlo);
+    k_layer_setup<<<grid_for(n, 4096), kBlock, 0, s>>>(d, parity_next);
+    k_rebuild<<<cb, kBlock, 0, s>>>(d, 1 - parity_next, 0, blocks, 0);
 }
 void launch_rewake(const DomainView &d, int parity_next, int blocks, hipStream_t s) {
     const int cb = rebuild_blocks(d, blocks);
@@ -1631,8 +1800,21 @@ void launch_grain_stats(const DomainView &d, const uint8_t *melted, const GrainR
     const long long total = static_cast<long long>(r.i1 - r.i0 + 1) * (r.j1 - r.j0 + 1) * (r.k1 - r.k0 + 1);
     k_grain_stats<<<grid_for(total, 2048), kBlock, 0, s>>>(d, melted, r, gmin, ngid, gflag, counts, obins, otop);
 }
+void launch_init_directional(const DomainView &d, const int32_t *plane_cts, int nx, int ny, uint8_t *melted,
+                             const uint32_t *substrate, uint32_t nsub, const int32_t *sites, uint32_t nsites,
+                             hipStream_t s) {
+    k_init_directional<<<grid_for(d.ncell, 1 << 16), kBlock, 0, s>>>(d, plane_cts, nx, ny, melted);
+    if (nsub) k_init_substrate<<<grid_for(nsub, 4096), kBlock, 0, s>>>(d, substrate, nsub);
+    if (nsites) k_init_sites<<<grid_for(nsites, 4096), kBlock, 0, s>>>(d, sites, nsites);
+}
 void launch_extract(const DomainView &d, int field, void *out, hipStream_t s) {
     k_extract<<<grid_for(d.ncell, 1 << 16), kBlock, 0, s>>>(d, field, out);
+}
+void launch_count_flags(const uint32_t *gflag, uint32_t n, unsigned long long *out, hipStream_t s) {
+    k_count_flags<<<grid_for(n, 1024), kBlock, 0, s>>>(gflag, n, out);
+}
+void launch_fill(uint32_t *p, size_t n, uint32_t v, hipStream_t s) {
+    k_fill<<<grid_for(static_cast<long long>(n), 4096), kBlock, 0, s>>>(p, n, v);
 }
 void launch_copy(const uint4 *src, uint4 *dst, size_t n, hipStream_t s) {
     k_copy<<<grid_for(static_cast<long long>(n), 1 << 16), kBlock, 0, s>>>(src, dst, n);

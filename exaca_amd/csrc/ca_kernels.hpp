@@ -18,6 +18,11 @@ void launch_apply(const DomainView &d, int parity, int layer, int blocks, hipStr
 // Every active cell due at the next step, and that step's list rebuilt: after a cycle jump, whose
 // skipped steps the calendar did not count. parity_next is the parity of the next step.
 void launch_rewake(const DomainView &d, int parity_next, int blocks, hipStream_t s);
+// The next layer's window: out[0] = least z of an active cell, out[1] = greatest z of a cell of
+// next_layer (out initialised to INT32_MAX, -1 by the caller).
+void launch_layer_bounds(const DomainView &d, int next_layer, int *out, hipStream_t s);
+// LayerSetup on the resident state for the window [d.zlo, d.nzA), then the first step's list.
+void launch_layer_setup(const DomainView &d, int parity_next, int blocks, hipStream_t s);
 // One step of every domain in a device-resident table of `slots` views (grid row = slot).
 void launch_step_n(const DomainView *views, int slots, int parity, int layer, int nuc_blocks, int blocks,
                    hipStream_t s);
@@ -34,11 +39,18 @@ void launch_step_timed(const DomainView &d, int parity, int layer, int blocks, h
 void launch_pack(const DomainView &d, const uint8_t *ct, const int8_t *lid, const int32_t *gid, const int32_t *cts,
                  hipStream_t s);
 void launch_extract(const DomainView &d, int field, void *out, hipStream_t s);
+// The initial state of a directional-solidification domain built on the device from its per-plane
+// CritTimeStep, the substrate's active cells and the nucleation sites (both in index order); the
+// Melted flags go to `melted` (one byte per cell).
+void launch_init_directional(const DomainView &d, const int32_t *plane_cts, int nx, int ny, uint8_t *melted,
+                             const uint32_t *substrate, uint32_t nsub, const int32_t *sites, uint32_t nsites,
+                             hipStream_t s);
 // Bytes per cell of a field as launch_extract writes it (0: not a per-cell field).
 inline size_t field_bytes(int field) {
     switch (field) {
     case kFieldCellTypeRaw: return 1;
     case kFieldCentre: return 12;
+    case kFieldCentreF64: return 24;
     case kFieldCellType:
     case kFieldGrainId:
     case kFieldDiagonal:
@@ -63,6 +75,10 @@ void launch_grain_stats(const DomainView &d, const uint8_t *melted, const GrainR
 uint32_t rebuild_resident(int cus);
 uint32_t rebuild_units(long long ncell);
 constexpr uint32_t kEnsembleUnits = 128;
+// Unique GrainIDs of a grain-statistics pass: the nonzero entries of gflag[0..n), added to *out.
+void launch_count_flags(const uint32_t *gflag, uint32_t n, unsigned long long *out, hipStream_t s);
+// p[0..n) = v, as a kernel (no memset node in a captured graph).
+void launch_fill(uint32_t *p, size_t n, uint32_t v, hipStream_t s);
 // Plain copy of n 16-byte words (HBM calibration).
 void launch_copy(const uint4 *src, uint4 *dst, size_t n, hipStream_t s);
 // Device-side self test of the capture geometry against host values: one octahedron per input row.

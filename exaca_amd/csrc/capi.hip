@@ -42,6 +42,8 @@ StepCtl initial_ctl(const HostDomain &hd) {
         c.ehi = std::max(c.ehi, q);
     }
     c.nactive = hd.active.size();
+    c.layer_zlo[0] = hd.zlow;
+    c.layer_zhi[0] = hd.zhigh;
     c.next_nuc[0] = 0x7FFFFFFF;
     c.next_nuc[1] = 0x7FFFFFFF;
     for (int32_t t : hd.ntime) c.next_nuc[0] = std::min(c.next_nuc[0], t);
@@ -53,8 +55,27 @@ StepCtl initial_ctl(const HostDomain &hd) {
 // packed on the device. The initially active cells are the first step's list (list[0]), with their
 // cell-centred octahedra; the calendar starts empty. Synchronises the stream (the host sources
 // are temporaries).
-void upload_cells(const DomainView &d, const HostDomain &hd, hipStream_t s) {
+void upload_cells(const DomainView &d, const HostDomain &hd, uint8_t *melted, hipStream_t s) {
     const size_t n = static_cast<size_t>(hd.ncell);
+    if (hd.sparse) {
+        // built on the device from the per-plane CritTimeStep and the draws' lists, staged in
+        // buffers that are free until the first step; the active list is the first step's list
+        int32_t *plane_cts = reinterpret_cast<int32_t *>(d.claim_bits);
+        int32_t *sites = reinterpret_cast<int32_t *>(d.claim_cell);
+        HIP_OK(hipMemcpyAsync(plane_cts, hd.plane_cts.data(), 4 * hd.plane_cts.size(), hipMemcpyHostToDevice, s));
+        if (!hd.nloc.empty())
+            HIP_OK(hipMemcpyAsync(sites, hd.nloc.data(), 4 * hd.nloc.size(), hipMemcpyHostToDevice, s));
+        if (!hd.active.empty())
+            HIP_OK(hipMemcpyAsync(d.list[0], hd.active.data(), 4 * hd.active.size(), hipMemcpyHostToDevice, s));
+        launch_init_directional(d, plane_cts, hd.nx, hd.ny, melted, d.list[0], static_cast<uint32_t>(hd.active.size()),
+                                sites, static_cast<uint32_t>(hd.nloc.size()), s);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemsetAsync(d.alarms, 0, 4 * kAlarmSlots * alarm_stride(d.ncell), s));
+        HIP_OK(hipMemsetAsync(d.chunk_flag, 0, 8ull * d.nchunks, s));
+        HIP_OK(hipStreamSynchronize(s));
+        return;
+    }
+    if (melted) HIP_OK(hipMemcpyAsync(melted, hd.melted.data(), n, hipMemcpyHostToDevice, s));
     std::vector<uint8_t> ct(hd.ct.begin(), hd.ct.end());
     for (uint32_t q : hd.active) ct[q] |= kSeedBit;
     uint8_t *ct_d = reinterpret_cast<uint8_t *>(d.claim_cell);
@@ -127,6 +148,11 @@ struct exaca_state {
     hipGraphExec_t graph = nullptr;
     int graph_layer = -1;
     double init_seconds = 0;
+    int device = 0;
+    int layer = 0;              // the layer being solidified
+    uint8_t *melted = nullptr;  // the reference's Melted flag per cell (grain statistics)
+    int gmin = 0;               // least initial GrainID and the size of the GrainID range
+    uint32_t ngid = 0;          //   (0: not yet found)
 
     template <typename T>
     T *alloc(size_t n) {
@@ -151,6 +177,7 @@ void build_device_state(exaca_state *h) {
     HIP_OK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     int dev = 0;
     HIP_OK(hipGetDevice(&dev));
+    h->device = dev;
     hipDeviceProp_t prop;
     HIP_OK(hipGetDeviceProperties(&prop, dev));
     // Frontier kernels: up to 4 workgroups per CU (the capture pass is then resident in one round;
@@ -222,7 +249,8 @@ void build_device_state(exaca_state *h) {
     HIP_OK(hipMemcpyAsync(guv, hd.guv.data(), hd.guv.size() * 4, hipMemcpyHostToDevice, s));
     HIP_OK(hipMemcpyAsync(seed_d, seed.data(), seed.size() * 4, hipMemcpyHostToDevice, s));
     HIP_OK(hipMemcpyAsync(grow_d, grow.data(), grow.size() * 8, hipMemcpyHostToDevice, s));
-    upload_cells(d, hd, s);
+    h->melted = h->alloc<uint8_t>(n);
+    upload_cells(d, hd, h->melted, s);
     StepCtl c = initial_ctl(hd);
     HIP_OK(hipMemcpyAsync(h->ctl, &c, sizeof c, hipMemcpyHostToDevice, s));
     HIP_OK(hipMemsetAsync(h->stats, 0, sizeof(StepStats), s));
@@ -249,6 +277,63 @@ void set_next_cycle(exaca_state *h, int next) {
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(h->stream));  // host values above are stack temporaries
     h->cycle = next - 1;
+}
+
+// The transition to the next layer (src/runCA.cpp:343-418): DomainShiftAndResize's new window
+// (src/CAinitialize.cpp:2772-2832), LayerSetup's cell-centred octahedra for the window's active
+// cells (:2835-2982), and the cycle count restarting at 0. The finished layer's last cycle and cycle
+// jumps stay on record: the undercooling of a cell carried into the new layer keeps counting its
+// earlier layers' steps.
+void next_layer(exaca_state *h, int layer_done, int verbose) {
+    using namespace exaca;
+    if (h->pending >= 0) throw StateError("layer change between nucleation and capture");
+    if (layer_done != h->layer) throw StateError("layer " + std::to_string(layer_done) + " is not the current layer");
+    if (layer_done + 1 >= h->deck.num_layers) throw StateError("no layer after the last one");
+    if (layer_done + 1 >= kMaxLayers) throw StateError("too many layers");
+    HIP_OK(hipStreamSynchronize(h->stream));
+    int bounds[2] = {0x7FFFFFFF, -1};
+    int *bounds_d = reinterpret_cast<int *>(h->check);  // the check's result block is free between steps
+    HIP_OK(hipMemcpyAsync(bounds_d, bounds, sizeof bounds, hipMemcpyHostToDevice, h->stream));
+    launch_layer_bounds(h->dv, layer_done + 1, bounds_d, h->stream);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(bounds, bounds_d, sizeof bounds, hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    const int zlo = bounds[0] - 1, zhigh = bounds[1];
+    if (bounds[0] == 0x7FFFFFFF || zhigh < 0 || zlo < 0 || zhigh < zlo || zhigh >= h->dv.nz)
+        throw StateError("no active window for layer " + std::to_string(layer_done + 1));
+    if (verbose) {
+        std::cout << "New domain bottom at Z = " << zlo << ", a shift of " << zlo - h->dv.zlo << " cells" << std::endl;
+        std::cout << "New domain top at Z = " << zhigh << std::endl;
+        std::cout << "New active domain height is " << zhigh - zlo + 1 << std::endl;
+    }
+    h->dv.zlo = zlo;
+    h->dv.nzA = zhigh + 1;
+    if (h->graph) {  // the graph holds the old window in its kernel arguments
+        HIP_OK(hipGraphExecDestroy(h->graph));
+        h->graph = nullptr;
+    }
+    // the layer's record, the new layer's first cycle, and fresh counts of the active cells
+    StepCtl *c = h->ctl;
+    const int32_t end = h->cycle, j0 = h->njumps, layer = layer_done + 1, one = 1, zero = 0, open = 0;
+    const uint32_t range[2] = {0xFFFFFFFFu, 0};
+    const unsigned long long none[2] = {0, 0};
+    HIP_OK(hipMemcpyAsync(&c->layer_end[layer_done], &end, 4, hipMemcpyHostToDevice, h->stream));
+    HIP_OK(hipMemcpyAsync(&c->layer_jump0[layer], &j0, 4, hipMemcpyHostToDevice, h->stream));
+    HIP_OK(hipMemcpyAsync(&c->layer, &layer, 4, hipMemcpyHostToDevice, h->stream));
+    HIP_OK(hipMemcpyAsync(&c->layer_zlo[layer], &zlo, 4, hipMemcpyHostToDevice, h->stream));
+    HIP_OK(hipMemcpyAsync(&c->layer_zhi[layer], &zhigh, 4, hipMemcpyHostToDevice, h->stream));
+    HIP_OK(hipMemcpyAsync(&c->cycle[h->parity], &one, 4, hipMemcpyHostToDevice, h->stream));
+    HIP_OK(hipMemcpyAsync(&c->last_cycle, &zero, 4, hipMemcpyHostToDevice, h->stream));
+    HIP_OK(hipMemcpyAsync(&c->next_nuc[h->parity], &open, 4, hipMemcpyHostToDevice, h->stream));
+    HIP_OK(hipMemcpyAsync(&c->elo, range, sizeof range, hipMemcpyHostToDevice, h->stream));
+    HIP_OK(hipMemcpyAsync(&c->nactive, none, sizeof none, hipMemcpyHostToDevice, h->stream));
+    // the calendar starts empty: every active cell of the window is due at the first step
+    launch_fill(h->dv.alarms, static_cast<size_t>(kAlarmSlots) * alarm_stride(h->dv.ncell), 0u, h->stream);
+    launch_layer_setup(h->dv, h->parity, h->blocks, h->stream);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(h->stream));  // host values above are stack temporaries
+    h->layer = layer;
+    h->cycle = 0;
 }
 
 void step_nucleation(exaca_state *h, int layer) {
@@ -346,6 +431,8 @@ void device_check(exaca_state *h, int *cycle, int layer, uint64_t counts[5], int
 
 void run_layer(exaca_state *h, int layer, int max_cycles, int verbose, int *out_cycle, double *gpu_ms) {
     if (h->pending >= 0) throw StateError("run_layer called between nucleation and capture");
+    if (layer != h->layer) throw StateError("layer " + std::to_string(layer) + " is not the current layer " +
+                                            std::to_string(h->layer) + " (exaca_next_layer moves on)");
     const bool graphs = std::getenv("EXACA_NO_GRAPHS") == nullptr;
     hipEvent_t e0, e1;
     HIP_OK(hipEventCreate(&e0));
@@ -375,12 +462,23 @@ void run_layer(exaca_state *h, int layer, int max_cycles, int verbose, int *out_
     if (gpu_ms) *gpu_ms = ms;
 }
 
+// The writers read LayerID and Melted from the host state; a domain initialised on the device has
+// them only there (LayerID is 0 everywhere in a directional-solidification domain).
+void host_fields_for_output(exaca_state *h) {
+    exaca::HostDomain &hd = h->host;
+    if (!hd.sparse) return;
+    const size_t n = static_cast<size_t>(hd.ncell);
+    hd.lid.assign(n, 0);
+    hd.melted.resize(n);
+    HIP_OK(hipMemcpy(hd.melted.data(), h->melted, n, hipMemcpyDeviceToHost));
+}
+
 exaca_state *create_state(const std::string &input, const std::string &dd, bool verbose, uint64_t seed = 0) {
     auto t0 = std::chrono::steady_clock::now();
     auto h = std::make_unique<exaca_state>();
     h->deck = exaca::read_deck(input, dd);
     h->deck.seed = seed;
-    h->host = exaca::initialize(h->deck, verbose);
+    h->host = exaca::initialize_sparse(h->deck, verbose);
     build_device_state(h.get());
     h->init_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return h.release();
@@ -434,6 +532,92 @@ void profile(exaca_state *h, int layer, int nsteps, double *ms, int64_t counts[5
     *launches = done;
 }
 
+// ---- Grain statistics (analysis/src/GAprint.cpp:17-108): one device pass over a region into
+// caller-provided buffers, then the host reduction of its counters and orientation bins. ----
+struct StatsBuffers {
+    uint32_t *gflag;          // one flag per GrainID of the initial range
+    unsigned long long *cnt;  // melted cells, nucleated cells, cells outside the range, unique GrainIDs
+    unsigned int *bins;       // melted cells per orientation: the region, then its top plane
+};
+
+exaca::GrainRegion checked_region(const exaca_state *h, const int32_t region[6]) {
+    const exaca::HostDomain &hd = h->host;
+    const int x0 = region[0], x1 = region[1], y0 = region[2], y1 = region[3], z0 = region[4], z1 = region[5];
+    if (x0 < 0 || x1 >= hd.nx || x0 > x1 || y0 < 0 || y1 >= hd.ny || y0 > y1 || z0 < 0 || z1 >= hd.nz || z0 > z1)
+        throw std::invalid_argument("region outside the domain");
+    return exaca::GrainRegion{z0, z1, x0, x1, y0, y1};
+}
+
+// Every GrainID of a run is an initial one (captures copy them) or 0: the flag range is the
+// initial field's, found once per handle.
+void grain_id_range(exaca_state *h) {
+    if (h->ngid) return;
+    int gmin = 0, gmax = 0;
+    if (h->host.sparse) {  // substrate grains 1 .. n, nuclei -1 .. -P
+        gmin = -static_cast<int>(h->host.nloc.size());
+        gmax = static_cast<int>(h->host.active.size());
+    }
+    for (int32_t g : h->host.gid) {
+        gmin = g < gmin ? g : gmin;
+        gmax = g > gmax ? g : gmax;
+    }
+    h->gmin = gmin;
+    h->ngid = static_cast<uint32_t>(static_cast<long long>(gmax) - gmin + 1);
+}
+
+void grain_stats_pass(exaca_state *h, const exaca::GrainRegion &r, const StatsBuffers &b) {
+    // cleared by a fill kernel, not a memset node (a caller may capture these launches)
+    exaca::launch_fill(b.gflag, h->ngid, 0u, h->stream);
+    exaca::launch_fill(reinterpret_cast<uint32_t *>(b.cnt), 8, 0u, h->stream);
+    exaca::launch_fill(b.bins, 2ull * exaca::kNumOrientations, 0u, h->stream);
+    exaca::launch_grain_stats(h->dv, h->melted, r, h->gmin, h->ngid, b.gflag, b.cnt, b.bins,
+                              b.bins + exaca::kNumOrientations, h->stream);
+    exaca::launch_count_flags(b.gflag, h->ngid, b.cnt + 3, h->stream);
+    HIP_OK(hipGetLastError());
+}
+
+// The printed quantities from the device results c[4] and ob[2 * kNumOrientations] (no heap use).
+void grain_stats_finish(const exaca_state *h, const int32_t region[6], const unsigned long long c[4],
+                        const unsigned int *ob, int64_t counts[5], double values[4], int64_t hist[64]) {
+    const exaca::HostDomain &hd = h->host;
+    if (c[2]) throw DeviceError("a GrainID outside the initial range");
+    const long long grains = static_cast<long long>(c[3]);
+    // per orientation: the least angle of its three axes to +Z, capped at 62.7 degrees
+    // (analysis/src/GAprint.cpp:38-47: stored as float; src/CAprint.cpp:423-430: rounded)
+    long double sum = 0, sum_top = 0;
+    long long top = 0;
+    int64_t hb[64] = {};
+    for (int o = 0; o < exaca::kNumOrientations; o++) {
+        double amin = 62.7;
+        for (int ll = 0; ll < 3; ll++) {
+            const double a = std::abs((180 / M_PI) * std::acos(static_cast<double>(hd.guv[9 * o + 3 * ll + 2])));
+            if (a < amin) amin = a;
+        }
+        const float af = static_cast<float>(amin);
+        sum += static_cast<long double>(ob[o]) * af;
+        sum_top += static_cast<long double>(ob[exaca::kNumOrientations + o]) * af;
+        top += ob[exaca::kNumOrientations + o];
+        hb[static_cast<int>(std::round(amin))] += ob[o];
+    }
+    const long long vol = static_cast<long long>(region[1] - region[0] + 1) * (region[3] - region[2] + 1) *
+                          (region[5] - region[4] + 1);
+    const long long nmelt = static_cast<long long>(c[0]);
+    if (counts) {
+        counts[0] = vol;
+        counts[1] = nmelt;
+        counts[2] = grains;
+        counts[3] = static_cast<int64_t>(c[1]);
+        counts[4] = top;
+    }
+    if (values) {
+        values[0] = nmelt ? static_cast<double>(sum / nmelt) : 0.0;
+        values[1] = top ? static_cast<double>(sum_top / top) : 0.0;
+        values[2] = h->deck.deltax * h->deck.deltax * std::pow(10.0, 12) * (static_cast<double>(vol) / grains);
+        values[3] = static_cast<double>(c[1]) / static_cast<double>(vol);
+    }
+    if (hist) std::copy(hb, hb + 64, hist);
+}
+
 }  // namespace
 
 extern "C" {
@@ -460,6 +644,7 @@ int exaca_nucleation(exaca_state *h, int cycle, int layer) {
     return guarded([&] {
         if (!h) throw std::invalid_argument("null handle");
         if (h->pending >= 0) throw StateError("nucleation called twice without capture");
+        if (layer != h->layer) throw StateError("layer " + std::to_string(layer) + " is not the current layer");
         set_next_cycle(h, cycle);
         step_nucleation(h, layer);
         HIP_OK(hipGetLastError());
@@ -496,6 +681,23 @@ int exaca_run_layer(exaca_state *h, int layer, int max_cycles, int verbose, int 
 int exaca_download(exaca_state *h, int field, void *dst, size_t dst_bytes) {
     return guarded([&] {
         if (!h || !dst) throw std::invalid_argument("null argument");
+        if (field == EXACA_FIELD_MELTED) {  // int32 per cell, from the device's flags
+            const size_t n = static_cast<size_t>(h->host.ncell);
+            if (dst_bytes < 4 * n) throw std::invalid_argument("destination too small");
+            std::vector<uint8_t> m(n);
+            HIP_OK(hipStreamSynchronize(h->stream));
+            HIP_OK(hipMemcpy(m.data(), h->melted, n, hipMemcpyDeviceToHost));
+            int32_t *o = static_cast<int32_t *>(dst);
+            for (size_t q = 0; q < n; q++) o[q] = m[q];
+            return;
+        }
+        if (field == EXACA_FIELD_NUCLEI_LOC || field == EXACA_FIELD_NUCLEI_TIME) {
+            // the sites as initialised (the device marks decided sites in its own copy)
+            const std::vector<int32_t> &v = field == EXACA_FIELD_NUCLEI_LOC ? h->host.nloc : h->host.ntime;
+            if (dst_bytes < 4 * v.size()) throw std::invalid_argument("destination too small");
+            if (!v.empty()) std::memcpy(dst, v.data(), 4 * v.size());
+            return;
+        }
         exaca::download_field(h->dv, field, dst, dst_bytes, h->stream);
     });
 }
@@ -504,90 +706,40 @@ int exaca_grain_stats(exaca_state *h, const int32_t region[6], int64_t counts[5]
     return guarded([&] {
         if (!h || !region) throw std::invalid_argument("null argument");
         if (h->pending >= 0) throw StateError("grain_stats called between nucleation and capture");
-        const exaca::HostDomain &hd = h->host;
-        const int x0 = region[0], x1 = region[1], y0 = region[2], y1 = region[3], z0 = region[4], z1 = region[5];
-        if (x0 < 0 || x1 >= hd.nx || x0 > x1 || y0 < 0 || y1 >= hd.ny || y0 > y1 || z0 < 0 || z1 >= hd.nz || z0 > z1)
-            throw std::invalid_argument("region outside the domain");
-        // every GrainID of the run is an initial one (captures copy them) or 0
-        int gmin = 0, gmax = 0;
-        for (int32_t g : hd.gid) {
-            gmin = g < gmin ? g : gmin;
-            gmax = g > gmax ? g : gmax;
-        }
-        const uint32_t ngid = static_cast<uint32_t>(gmax - gmin + 1);
-        const size_t n = static_cast<size_t>(hd.ncell);
-        uint8_t *melted = nullptr;
-        uint32_t *gflag = nullptr;
-        unsigned long long *cnt = nullptr;
-        unsigned int *bins = nullptr;
+        const exaca::GrainRegion r = checked_region(h, region);
+        grain_id_range(h);
+        // device buffers of this call: GrainID flags, the counters and the orientation bins
         struct Free {
             std::vector<void *> p;
             ~Free() {
                 for (void *q : p) (void)hipFree(q);
             }
         } tmp;
-        HIP_OK(hipMalloc(&melted, n));
-        tmp.p.push_back(melted);
-        HIP_OK(hipMalloc(&gflag, 4ull * ngid));
-        tmp.p.push_back(gflag);
-        HIP_OK(hipMalloc(&cnt, 8 * 4));
-        tmp.p.push_back(cnt);
-        HIP_OK(hipMalloc(&bins, 4ull * 2 * exaca::kNumOrientations));
-        tmp.p.push_back(bins);
-        HIP_OK(hipMemcpyAsync(melted, hd.melted.data(), n, hipMemcpyHostToDevice, h->stream));
-        HIP_OK(hipMemsetAsync(gflag, 0, 4ull * ngid, h->stream));
-        HIP_OK(hipMemsetAsync(cnt, 0, 8 * 4, h->stream));
-        HIP_OK(hipMemsetAsync(bins, 0, 4ull * 2 * exaca::kNumOrientations, h->stream));
-        const exaca::GrainRegion r{z0, z1, x0, x1, y0, y1};
-        exaca::launch_grain_stats(h->dv, melted, r, gmin, ngid, gflag, cnt, bins, bins + exaca::kNumOrientations,
-                                  h->stream);
-        HIP_OK(hipGetLastError());
-        std::vector<uint32_t> flags(ngid), ob(2 * exaca::kNumOrientations);
+        StatsBuffers b{};
+        HIP_OK(hipMalloc(&b.gflag, 4ull * h->ngid));
+        tmp.p.push_back(b.gflag);
+        HIP_OK(hipMalloc(&b.cnt, 8 * 4));
+        tmp.p.push_back(b.cnt);
+        HIP_OK(hipMalloc(&b.bins, 4ull * 2 * exaca::kNumOrientations));
+        tmp.p.push_back(b.bins);
+        grain_stats_pass(h, r, b);
+        std::vector<unsigned int> ob(2 * exaca::kNumOrientations);
         unsigned long long c[4];
-        HIP_OK(hipMemcpyAsync(flags.data(), gflag, 4ull * ngid, hipMemcpyDeviceToHost, h->stream));
-        HIP_OK(hipMemcpyAsync(ob.data(), bins, 4ull * ob.size(), hipMemcpyDeviceToHost, h->stream));
-        HIP_OK(hipMemcpyAsync(c, cnt, sizeof c, hipMemcpyDeviceToHost, h->stream));
+        HIP_OK(hipMemcpyAsync(ob.data(), b.bins, 4ull * ob.size(), hipMemcpyDeviceToHost, h->stream));
+        HIP_OK(hipMemcpyAsync(c, b.cnt, sizeof c, hipMemcpyDeviceToHost, h->stream));
         HIP_OK(hipStreamSynchronize(h->stream));
-        if (c[2]) throw DeviceError("a GrainID outside the initial range");
-        long long grains = 0;
-        for (uint32_t f : flags) grains += f != 0;
-        // per orientation: the least angle of its three axes to +Z, capped at 62.7 degrees
-        // (analysis/src/GAprint.cpp:38-47: stored as float; src/CAprint.cpp:423-430: rounded)
-        long double sum = 0, sum_top = 0;
-        long long top = 0;
-        std::vector<int64_t> hb(64, 0);
-        for (int o = 0; o < exaca::kNumOrientations; o++) {
-            double amin = 62.7;
-            for (int ll = 0; ll < 3; ll++) {
-                const double a = std::abs((180 / M_PI) * std::acos(static_cast<double>(hd.guv[9 * o + 3 * ll + 2])));
-                if (a < amin) amin = a;
-            }
-            const float af = static_cast<float>(amin);
-            sum += static_cast<long double>(ob[o]) * af;
-            sum_top += static_cast<long double>(ob[exaca::kNumOrientations + o]) * af;
-            top += ob[exaca::kNumOrientations + o];
-            hb[static_cast<int>(std::round(amin))] += ob[o];
-        }
-        const long long vol = static_cast<long long>(x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 1);
-        const long long nmelt = static_cast<long long>(c[0]);
-        if (counts) {
-            counts[0] = vol;
-            counts[1] = nmelt;
-            counts[2] = grains;
-            counts[3] = static_cast<int64_t>(c[1]);
-            counts[4] = top;
-        }
-        if (values) {
-            values[0] = nmelt ? static_cast<double>(sum / nmelt) : 0.0;
-            values[1] = top ? static_cast<double>(sum_top / top) : 0.0;
-            values[2] = h->deck.deltax * h->deck.deltax * std::pow(10.0, 12) * (static_cast<double>(vol) / grains);
-            values[3] = static_cast<double>(c[1]) / static_cast<double>(vol);
-        }
-        if (hist) std::copy(hb.begin(), hb.end(), hist);
+        grain_stats_finish(h, region, c, ob.data(), counts, values, hist);
     });
 }
 
-int exaca_query(const exaca_state *h, int64_t out[18]) {
+int exaca_next_layer(exaca_state *h, int layer, int verbose) {
+    return guarded([&] {
+        if (!h) throw std::invalid_argument("null handle");
+        next_layer(h, layer, verbose);
+    });
+}
+
+int exaca_query(const exaca_state *h, int64_t out[20]) {
     return guarded([&] {
         if (!h || !out) throw std::invalid_argument("null argument");
         exaca::StepStats st;
@@ -614,6 +766,8 @@ int exaca_query(const exaca_state *h, int64_t out[18]) {
         out[15] = static_cast<int64_t>(h->bytes);
         out[16] = static_cast<int64_t>(st.evaluated);
         out[17] = static_cast<int64_t>(st.masks);
+        out[18] = h->deck.num_layers;
+        out[19] = h->layer;
     });
 }
 
@@ -744,7 +898,19 @@ int exaca_run_program(int id, int np, const char *input_file) {
         int cycle = 0;
         double ms = 0;
         auto t1 = std::chrono::steady_clock::now();
-        run_layer(h.get(), 0, 0, 1, &cycle, &ms);
+        const int layers = h->deck.num_layers;
+        for (int layer = 0; layer < layers; layer++) {
+            auto tl = std::chrono::steady_clock::now();
+            double lms = 0;
+            run_layer(h.get(), layer, 0, 1, &cycle, &lms);
+            ms += lms;
+            const double took = std::chrono::duration<double>(std::chrono::steady_clock::now() - tl).count();
+            if (layer + 1 < layers) {
+                next_layer(h.get(), layer, 1);
+                std::cout << "Time for layer number " << layer << " was " << took << " s, starting layer "
+                          << layer + 1 << std::endl;
+            }
+        }
         auto t2 = std::chrono::steady_clock::now();
         exaca::RunTimes times;
         times.init = std::chrono::duration<double>(t1 - t0).count();
@@ -756,6 +922,7 @@ int exaca_run_program(int id, int np, const char *input_file) {
             std::cout << "Collecting data on rank 0 and printing to files" << std::endl;
             std::vector<int32_t> gid(static_cast<size_t>(h->dv.ncell));
             exaca::download_field(h->dv, exaca::kFieldGrainId, gid.data(), gid.size() * 4, h->stream);
+            host_fields_for_output(h.get());
             exaca::write_vtk_files(d, h->host, gid);
         }
         else {

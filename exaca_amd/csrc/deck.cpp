@@ -183,9 +183,8 @@ Deck read_deck(const std::string &path, const std::string &dd) {
         bool b = yes_no(in, "Print data for all main data structures");
         d.print_debug = b ? 2 : (a ? 1 : 0);
     }
-    if (d.num_layers != 1)
-        throw std::runtime_error("multilayer problems (" + std::to_string(d.num_layers) +
-                                 " layers) are not provided by this build");
+    if (d.num_layers < 1 || d.num_layers > 255)
+        throw std::runtime_error("number of layers must be 1 .. 255 (" + std::to_string(d.num_layers) + ")");
     return d;
 }
 

@@ -24,7 +24,7 @@
 #include "runtime_util.hpp"
 
 namespace exaca {
-void upload_cells(const DomainView &d, const HostDomain &hd, hipStream_t s);
+void upload_cells(const DomainView &d, const HostDomain &hd, uint8_t *melted, hipStream_t s);
 StepCtl initial_ctl(const HostDomain &hd);
 void download_field(const DomainView &d, int field, void *dst, size_t dst_bytes, hipStream_t s);
 }  // namespace exaca
@@ -236,7 +236,7 @@ void admit(exaca_ensemble *e, const exaca_prepared *pr, int *slot_out) {
         HIP_OK(hipMemcpyAsync(ntime, hd.ntime.data(), 4 * hd.ntime.size(), hipMemcpyHostToDevice, s));
     }
     HIP_OK(hipMemcpyAsync(grow, pr->grow.data(), 8 * pr->grow.size(), hipMemcpyHostToDevice, s));
-    exaca::upload_cells(d, hd, s);
+    exaca::upload_cells(d, hd, nullptr, s);
     // Chunks end on parity 0, so a new domain always starts in slot 0 of its control block.
     StepCtl c = exaca::initial_ctl(hd);
     HIP_OK(hipMemcpyAsync(d.ctl, &c, sizeof c, hipMemcpyHostToDevice, s));
@@ -309,7 +309,7 @@ int exaca_prepare(const char *input_file, const char *data_dir, uint64_t seed, e
         if (pr->deck.sim_type != "C")
             throw std::invalid_argument("ensembles hold directional-solidification (type C) domains");
         pr->deck.seed = seed;
-        pr->host = exaca::initialize(pr->deck, false);
+        pr->host = exaca::initialize_sparse(pr->deck, false);
         const exaca::HostDomain &h = pr->host;
         pr->grow = exaca::growth_table(h.A, h.B, h.C, h.D, h.dU, 1 << 25);
         if (pr->grow.size() >= (1u << 25)) throw std::runtime_error("undercooling never saturates within 2^25 steps");
@@ -317,7 +317,10 @@ int exaca_prepare(const char *input_file, const char *data_dir, uint64_t seed, e
             throw std::invalid_argument("domain too large for an ensemble slot (at most " +
                                         std::to_string(exaca::kEnsembleUnits) +
                                         " list-rebuild units, about 33 M cells); run it on the plain path");
-        for (long long q = 0; q < h.ncell; q++) pr->liquid0 += (h.ct[q] == exaca::kLiquid && h.lid[q] == 0);
+        if (h.sparse)  // every cell inside the walls is liquid but the substrate's active cells
+            pr->liquid0 = static_cast<long long>(h.nx) * h.ny * (h.nz - 2) - static_cast<long long>(h.active.size());
+        else
+            for (long long q = 0; q < h.ncell; q++) pr->liquid0 += (h.ct[q] == exaca::kLiquid && h.lid[q] == 0);
         pr->bytes = domain_bytes(h, pr->grow.size());
         *out = pr.release();
     });

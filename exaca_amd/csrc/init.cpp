@@ -531,6 +531,83 @@ int growth_plateau(const std::vector<double> &grow) {
     return k < 0 ? 0 : k;
 }
 
+HostDomain initialize_sparse(const Deck &d, bool verbose) {
+    if (d.sim_type != "C" || d.nx < 2 || d.ny < 2 || d.nz < 3) return initialize(d, verbose);
+    HostDomain h;
+    h.nx = d.nx;
+    h.ny = d.ny;
+    h.nz = d.nz;
+    h.X = d.nx + 2;
+    h.Y = d.ny + 2;
+    h.xoff = -1;
+    h.yoff = -1;
+    h.ncell = static_cast<long long>(h.X) * h.Y * h.nz;
+    if (h.ncell >= (1LL << 32) - 1) throw std::runtime_error("domain exceeds 2^32 cells");
+    h.sparse = true;
+    h.zlow = 0;
+    h.zhigh = h.nz - 1;
+    h.nzA = h.nz;
+    h.dU = static_cast<float>(d.R * d.deltat);
+    h.plane_cts.resize(h.nz);
+    for (int k = 0; k < h.nz; k++) h.plane_cts[k] = crit_directional(d, k);
+    // GrainInit's draws (one per interior cell of a plane with melting data, (z, y, x) order) on
+    // their own thread while the substrate's draws run here
+    NucleusDraws nd;
+    nd.first = draws_before_planes_directional(d, h);
+    const double bulk = d.nmax * d.deltax * d.deltax * d.deltax;
+    std::thread drawer([&] { nd.hits = draws_below(d.seed, bulk, nd.first[std::max(h.nz - 1, 1)]); });
+    struct JoinOnExit {
+        std::thread &t;
+        ~JoinOnExit() {
+            if (t.joinable()) t.join();
+        }
+    } join_drawer{drawer};
+    // SubstrateInit_ConstrainedGrowth: one draw per interior cell of plane 1, in index order
+    {
+        std::mt19937_64 gen(d.seed);
+        std::uniform_real_distribution<double> dis(0.0, 1.0);
+        for (int i = 1; i < h.X - 1; i++)
+            for (int j = 1; j < h.Y - 1; j++)
+                if (dis(gen) < d.fract_active) h.active.push_back(static_cast<uint32_t>(cell(h, 1, i, j)));
+        h.substrate_active = static_cast<int>(h.active.size());
+    }
+    h.guv = read_orientations(d.orientation_file);
+    drawer.join();
+    // the marked cells: draw c of plane k is interior cell (i, j) with c - first[k] = (j - 1) (X - 2) + i - 1
+    const long long row = h.X - 2;
+    std::vector<uint32_t> sites;
+    sites.reserve(nd.hits.size());
+    for (size_t a = 0; a < nd.hits.size();) {
+        int k = static_cast<int>(std::upper_bound(nd.first.begin(), nd.first.end(), nd.hits[a]) - nd.first.begin()) - 1;
+        const size_t b0 = sites.size();
+        for (; a < nd.hits.size() && nd.hits[a] < nd.first[k + 1]; a++) {
+            const long long c = nd.hits[a] - nd.first[k];
+            sites.push_back(static_cast<uint32_t>(cell(h, k, static_cast<int>(c % row) + 1, static_cast<int>(c / row) + 1)));
+        }
+        std::sort(sites.begin() + b0, sites.end());  // draw order is (y, x), index order (x, y)
+    }
+    // NucleiInit: one normal draw per site in index order
+    std::default_random_engine generator(static_cast<std::default_random_engine::result_type>(d.seed % 2147483646u + 1u));
+    std::normal_distribution<double> distribution(d.dTN, d.dTsigma);
+    const long long plane_cells = static_cast<long long>(h.X) * h.Y;
+    for (const uint32_t p : sites) {
+        const int cts = h.plane_cts[p / plane_cells];
+        const double und = distribution(generator);
+        const int t = static_cast<int>(cts + std::round(und / h.dU));
+        h.nloc.push_back(static_cast<int32_t>(p));
+        h.ntime.push_back(std::max(cts, t));
+    }
+    h.A = d.A * d.deltat / d.deltax;
+    h.B = d.B * d.deltat / d.deltax;
+    h.C = d.C * d.deltat / d.deltax;
+    h.D = d.D;
+    if (verbose) {
+        std::cout << "Number of substrate active cells: " << h.substrate_active << std::endl;
+        std::cout << "Number of potential nucleated grains: " << h.nloc.size() << std::endl;
+    }
+    return h;
+}
+
 HostDomain initialize(const Deck &d, bool verbose) {
     HostDomain h;
     h.nx = d.nx;

@@ -49,6 +49,12 @@ struct HostDomain {
     std::vector<int32_t> nloc, ntime;  // nucleation sites and steps (NucleiLocation/NucleationTimes)
     std::vector<float> guv;       // GrainUnitVector, 10000 x 9
     std::vector<uint32_t> active; // initially active cells, ascending index
+    // Directional-solidification domains initialised on the device (initialize_sparse): the
+    // whole-domain fields above stay empty; the device builds them from the deck, the per-plane
+    // CritTimeStep and these lists (active = the substrate's active cells, numbered 1, 2, ... in
+    // index order; nloc = the nucleation sites, numbered -1, -2, ... in index order).
+    bool sparse = false;
+    std::vector<int32_t> plane_cts;  // CritTimeStep of each plane
     // normalised interfacial response (A, B, C scaled by deltat/deltax; D unscaled, runCA.cpp:198-200)
     double A = 0, B = 0, C = 0, D = 0;
     int substrate_active = 0, baseplate_grains = 0;
@@ -57,6 +63,14 @@ struct HostDomain {
 // Runs TempInit_{DirSolidification,SpotMelt}, OrientationInit, the substrate init for the deck's
 // problem type, ActiveCellWallInit (type S), GrainInit and NucleiInit for rank 0 of 1.
 HostDomain initialize(const Deck &deck, bool verbose);
+
+// The same initial state of a directional-solidification deck without its whole-domain fields:
+// only the random draws run on the host (the substrate's active cells, GrainInit's nucleation
+// sites, NucleiInit's undercoolings), in the reference's streams and orders; the device builds every
+// cell from them (launch_init_directional). Decks of other types, and domains narrower than two
+// interior cells in x or y (where a liquid cell could lack any non-wall neighbour and GrainInit's
+// last pass would turn it solid), return a full host state instead (sparse == false).
+HostDomain initialize_sparse(const Deck &deck, bool verbose);
 
 // The 26 critical diagonal lengths of a cell-centred octahedron of one orientation
 // (src/CAupdate.cpp:112-194), evaluated with the host libm exactly as the reference does.

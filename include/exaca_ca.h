@@ -39,7 +39,10 @@ enum exaca_field {
     EXACA_FIELD_CELLTYPE_RAW = 6, /* uint8, type | 0x10 for cell-centred octahedra */
     EXACA_FIELD_NUCLEI_LOC = 7,   /* int32 per nucleation site (NucleiLocation), host state only */
     EXACA_FIELD_NUCLEI_TIME = 8,  /* int32 per nucleation site (NucleationTimes), host state only */
-    EXACA_FIELD_MELTED = 9,       /* int32 per cell, the reference's Melted flag, host state only */
+    EXACA_FIELD_MELTED = 9,       /* int32 per cell, the reference's Melted flag */
+    EXACA_FIELD_CENTRE_F64 = 10,  /* double[3] per cell: the octahedron centre as computed at capture
+                                     (DOCenter is its float rounding); with CELLTYPE_RAW, GRAINID and
+                                     DIAGONAL it is the whole front state of a step */
 };
 
 /* Replaces the initialisation half of RunProgram_Reduced (src/runCA.cpp:37-256): reads the deck
@@ -74,14 +77,20 @@ int exaca_check(exaca_state *h, int *cycle, int layer, uint64_t counts[5], int *
 int exaca_run_layer(exaca_state *h, int layer, int max_cycles, int verbose, int *cycle, double *gpu_ms);
 
 /* Device -> host copy of one whole-domain field (kind: enum exaca_field) into dst of
- * dst_bytes bytes. Replaces the deep_copy at src/runCA.cpp:423-424. */
+ * dst_bytes bytes. Replaces the deep_copy at src/runCA.cpp:423-424. NUCLEI_LOC and NUCLEI_TIME
+ * give the handle's nucleation sites as initialised (int32 per site). */
 int exaca_download(exaca_state *h, int field, void *dst, size_t dst_bytes);
 
-/* Sizes and counters. out[0..17]: X, Y, nz, nzActive, nx, ny, nuclei, last cycle,
+/* Sizes and counters. out[0..19]: X, Y, nz, nzActive, nx, ny, nuclei, last cycle,
  * nucleation events (nn), sum of frontier sizes, captures, solidifications, waiting-entry steps,
  * frontier list length, frontier list capacity, device bytes allocated, critical-length
- * evaluations, liquid-neighbour masks built. */
-int exaca_query(const exaca_state *h, int64_t out[18]);
+ * evaluations, liquid-neighbour masks built, number of layers, current layer. */
+int exaca_query(const exaca_state *h, int64_t out[20]);
+
+/* The transition after layer `layer` (src/runCA.cpp:343-418): DomainShiftAndResize and LayerSetup
+ * (src/CAinitialize.cpp:2772-2982) on the resident state, and the cycle count restarting at 0.
+ * Call it when exaca_run_layer (or the check) has ended the layer; then run layer + 1. */
+int exaca_next_layer(exaca_state *h, int layer, int verbose);
 
 /* Grain statistics of a region, computed on the device from the current GrainID field: the
  * quantities of the reference's analysis executable (analysis/src/GAprint.cpp:17-108,

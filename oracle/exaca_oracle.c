@@ -189,6 +189,12 @@ struct oracle_sim {
     long long nn;
     uint64_t sum_front, captures, solidifications;
     int mutation; /* test hook: a planted mistake in pass 2 (ORACLE_MUT_*), 0 for the restatement */
+    int nthreads; /* > 1: CellCapture runs as the reference's OpenMP backend does (timing only) */
+    uint64_t seed; /* random-stream seed of a type-C initial state (0: the reference's own streams) */
+    int layer;     /* the layer being solidified (src/runCA.cpp:283) */
+    long long nn_layer; /* nucleation events before this layer (the reference's nn restarts per layer) */
+    long long outside;  /* nuclei or captures that fell outside the active window (the reference would
+                           index its window arrays out of bounds there); skipped and counted */
 };
 
 /* ---------------- deck parsing (src/CAinitialize.cpp:24-134) ---------------- */
@@ -418,7 +424,7 @@ static int read_deck(oracle_sim *s, const char *deck, const char *data_dir) {
         }
     }
     fclose(in);
-    if (s->nlayers != 1) return fail("multilayer decks (%d layers) are outside the restated path", s->nlayers);
+    if (s->nlayers < 1) return fail("at least one layer is needed");
     return 0;
 bad:
     if (mat) fclose(mat);
@@ -563,7 +569,7 @@ static int orient_init(oracle_sim *s) {
 /* ---------------- substrates (src/CAinitialize.cpp:1460-1732) ---------------- */
 static void substrate_C(oracle_sim *s) {
     mt64_t g;
-    mt64_seed(&g, 0); /* rank id 0 */
+    mt64_seed(&g, s->seed); /* rank id 0 (or the instance seed) */
     for (int k = 0; k < s->nz; k++)
         for (int i = 0; i < s->X; i++)
             for (int j = 0; j < s->Y; j++) {
@@ -773,7 +779,7 @@ static int orientation_of(const oracle_sim *s, int gid) {
 static int grain_init(oracle_sim *s) {
     const int X = s->X, Y = s->Y;
     mt64_t g;
-    mt64_seed(&g, 0);
+    mt64_seed(&g, s->seed);
     double bulk = s->NMax * s->dx * s->dx * s->dx;
     s->P = 0;
     for (int k = 1; k < s->nz - 1; k++)
@@ -834,7 +840,7 @@ static int grain_init(oracle_sim *s) {
 /* NucleiInit (src/CAinitialize.cpp:2213-2248, 2764-2767), np = 1. */
 static int nuclei_init(oracle_sim *s) {
     normal_t nd;
-    nd.g.x = 1u;
+    nd.g.x = (uint32_t)(s->seed % 2147483646u + 1u); /* default seed 1 (or from the instance seed) */
     nd.have = 0;
     nd.saved = 0;
     s->nloc = calloc((size_t)(s->P > 0 ? s->P : 1), 4);
@@ -854,11 +860,115 @@ static int nuclei_init(oracle_sim *s) {
     return 0;
 }
 
-int oracle_load(const char *deck, const char *data_dir, oracle_sim **out) {
+static void critical_lengths(const float *g, double xp, double yp, double zp, double cx, double cy, double cz,
+                             const int *nbx, const int *nby, const int *nbz, float *cdl_out);
+
+/* A simulation whose cell state is taken from another run at the end of step `cycle`: the deck,
+ * mesh, orientations and temperature fields are the oracle's own (TempInit sets CritTimeStep,
+ * LayerID, UndercoolingChange and Melted); CellType, GrainID, the nucleation sites and the active
+ * cells' octahedra come from the caller. UndercoolingCurrent is what the reference's pass 1
+ * accumulates (src/CAupdate.cpp:235-237): the float sum of cycle - CritTimeStep increments for a
+ * liquid or active cell past its liquidus time. The critical lengths are recomputed from the
+ * centre (double, as computed at capture) or, for a cell-centred octahedron (bit 0x10 of ct_raw),
+ * from the orientation (src/CAupdate.cpp:112-194, 524-563), as the reference computed them. Type
+ * C only (no cycle jump; one UndercoolingChange shared by every cell). */
+int oracle_load_state(const char *deck, const char *data_dir, int cycle, const uint8_t *ct_raw, const int32_t *gid,
+                      const float *dl, const double *centre, int P, const int32_t *nloc, const int32_t *ntime,
+                      oracle_sim **out) {
     oracle_sim *s = calloc(1, sizeof *s);
     if (!s) return fail("out of host memory");
     *out = NULL;
     if (read_deck(s, deck, data_dir)) goto bad;
+    if (strcmp(s->type, "C") != 0) {
+        fail("state import covers directional solidification (type C) decks");
+        goto bad;
+    }
+    mesh_init(s);
+    if (alloc_fields(s)) goto bad;
+    temp_init_C(s);
+    if (orient_init(s)) goto bad;
+    s->nact = (long long)s->X * s->Y * s->nzA;
+    s->DL = calloc((size_t)s->nact, 4);
+    s->DOC = calloc((size_t)s->nact * 3, 4);
+    s->CDL = calloc((size_t)s->nact * 26, 4);
+    s->steer = calloc((size_t)s->nact, 4);
+    s->P = P;
+    s->nloc = calloc((size_t)(P > 0 ? P : 1), 4);
+    s->ntime = calloc((size_t)(P > 0 ? P : 1), 4);
+    if (!s->DL || !s->DOC || !s->CDL || !s->steer || !s->nloc || !s->ntime) {
+        fail("out of host memory");
+        goto bad;
+    }
+    if (P > 0) {
+        memcpy(s->nloc, nloc, (size_t)P * 4);
+        memcpy(s->ntime, ntime, (size_t)P * 4);
+    }
+    {
+        /* U_k, the float sum of k increments of the shared UndercoolingChange */
+        const float du = (float)(s->R * s->dt);
+        int kmax = 0;
+        for (long long p = 0; p < s->ndom; p++) {
+            int k = cycle - s->cts[p];
+            if (k > kmax) kmax = k;
+        }
+        float *uk = calloc((size_t)kmax + 1, sizeof(float));
+        if (!uk) {
+            fail("out of host memory");
+            goto bad;
+        }
+        for (int k = 1; k <= kmax; k++) uk[k] = uk[k - 1] + du * (float)1;
+        const long long XY = (long long)s->X * s->Y;
+        for (long long p = 0; p < s->ndom; p++) {
+            const int t = ct_raw[p] & 0x0F;
+            s->ct[p] = t;
+            s->gid[p] = gid[p];
+            if ((t == LIQUID || t == ACTIVE) && cycle > s->cts[p]) s->U[p] = uk[cycle - s->cts[p]];
+            if (t != ACTIVE) continue;
+            const long long pa = p - (long long)s->zlow * XY;
+            const int rz = (int)(pa / XY);
+            const long long rem = pa % XY;
+            const int rx = (int)(rem / s->Y), ry = (int)(rem % s->Y);
+            const int o = orientation_of(s, gid[p]);
+            s->DL[pa] = dl[p];
+            s->DOC[3 * pa] = (float)centre[3 * p];
+            s->DOC[3 * pa + 1] = (float)centre[3 * p + 1];
+            s->DOC[3 * pa + 2] = (float)centre[3 * p + 2];
+            if (ct_raw[p] & 0x10)
+                seed_cdl(&s->guv[9 * o], s->nbx, s->nby, s->nbz, &s->CDL[26 * pa]);
+            else
+                critical_lengths(&s->guv[9 * o], (rx + s->xoff) + 0.5, (ry + s->yoff) + 0.5, (rz + s->zlow) + 0.5,
+                                 centre[3 * p], centre[3 * p + 1], centre[3 * p + 2], s->nbx, s->nby, s->nbz,
+                                 &s->CDL[26 * pa]);
+        }
+        free(uk);
+    }
+    s->A = s->A * s->dt / s->dx;
+    s->B = s->B * s->dt / s->dx;
+    s->C = s->C * s->dt / s->dx;
+    s->cycle = cycle;
+    *out = s;
+    return 0;
+bad:
+    oracle_free(s);
+    return -1;
+}
+
+int oracle_load(const char *deck, const char *data_dir, oracle_sim **out) {
+    return oracle_load_seeded(deck, data_dir, 0, out);
+}
+
+/* An ensemble instance's streams: mt19937_64(seed) for the substrate and the nucleation sites,
+ * default_random_engine(seed % 2147483646 + 1) for the nucleation undercoolings (type C). */
+int oracle_load_seeded(const char *deck, const char *data_dir, uint64_t seed, oracle_sim **out) {
+    oracle_sim *s = calloc(1, sizeof *s);
+    if (!s) return fail("out of host memory");
+    *out = NULL;
+    s->seed = seed;
+    if (read_deck(s, deck, data_dir)) goto bad;
+    if (seed && strcmp(s->type, "C") != 0) {
+        fail("seeded initial states cover directional solidification (type C) decks");
+        goto bad;
+    }
     mesh_init(s);
     if (alloc_fields(s)) goto bad;
     if (strcmp(s->type, "S") == 0)
@@ -907,7 +1017,12 @@ void oracle_free(oracle_sim *s) {
     free(s);
 }
 
-void oracle_dims(const oracle_sim *s, int64_t d[12]) {
+void oracle_dims(const oracle_sim *s, int64_t d[17]) {
+    d[16] = s->outside;
+    d[12] = s->nlayers;
+    d[13] = s->layer;
+    d[14] = s->zlow;
+    d[15] = s->zhigh;
     d[0] = s->X;
     d[1] = s->Y;
     d[2] = s->nz;
@@ -993,6 +1108,11 @@ static void nucleation(oracle_sim *s, int cycle, int layer) {
         int gz = (int)(pg / ((long long)X * Y));
         long long rem = pg % ((long long)X * Y);
         int rx = (int)(rem / Y), ry = (int)(rem % Y), rz = gz - s->zlow;
+        if (rz < 0 || rz >= s->nzA) {
+            s->ct[pg] = LIQUID;
+            s->outside++;
+            continue;
+        }
         long long pa = ((long long)rz * X + rx) * Y + ry;
         if (rx > 0 && rx < X - 1 && ry > 0 && ry < Y - 1) s->nn++;
         int o = orientation_of(s, s->gid[pg]);
@@ -1069,6 +1189,13 @@ static void capture_octahedron(const float *g, double cxold, double cyold, doubl
     c_out[0] = cx;
     c_out[1] = cy;
     c_out[2] = cz;
+    critical_lengths(g, xp, yp, zp, cx, cy, cz, nbx, nby, nbz, cdl_out);
+}
+/* The 26 critical diagonal lengths of an octahedron centred at (cx, cy, cz) around the cell
+ * centre (xp, yp, zp): the largest |F_q . (x_n - c)| over the four face normals
+ * (src/CAupdate.cpp:524-563), stored as float. */
+static void critical_lengths(const float *g, double xp, double yp, double zp, double cx, double cy, double cz,
+                             const int *nbx, const int *nby, const int *nbz, float *cdl_out) {
     double Fx[4], Fy[4], Fz[4];
     Fx[0] = g[0] + g[3] + g[6];
     Fx[1] = g[0] - g[3] + g[6];
@@ -1124,6 +1251,10 @@ static void cell_capture(oracle_sim *s, int cycle, int layer) {
             int l = s->itlist[b][ll];
             int mx = rx + s->nbx[l], my = ry + s->nby[l], mz = rz + s->nbz[l];
             if (mz >= s->nzA) continue;
+            if (mz < 0) { /* below the window: never liquid in a well-formed run */
+                if (s->ct[((long long)(mz + s->zlow) * X + mx) * Y + my] == LIQUID) s->outside++;
+                continue;
+            }
             long long na = ((long long)mz * X + mx) * Y + my;
             long long ng = ((long long)(mz + s->zlow) * X + mx) * Y + my;
             const int liquid = s->ct[ng] == LIQUID;
@@ -1157,14 +1288,145 @@ static void cell_capture(oracle_sim *s, int cycle, int layer) {
     }
 }
 
+/* CellCapture as the reference's Kokkos OpenMP backend executes it (src/CAupdate.cpp:204-728):
+ * pass 1 in parallel, each active cell appended to the steering vector with an atomic fetch-add
+ * (:238-240, so in no fixed order); pass 2 in parallel over the steering vector, a capture taken
+ * by compare-exchange of the neighbour's type from Liquid to TemporaryUpdate (:336-352), the
+ * neighbour types read while other threads update them (:333-334). Which of two cells reaching
+ * one liquid cell in a step captures it then depends on thread timing, so results are not
+ * reproducible: this is the timing baseline, never the parity checker. */
+static void cell_capture_omp(oracle_sim *s, int cycle, int layer) {
+    const int X = s->X, Y = s->Y;
+    const long long XY = (long long)X * Y;
+    long long nsteer = 0;
+    uint64_t ncap = 0, nsol = 0;
+#pragma omp parallel for num_threads(s->nthreads) schedule(static)
+    for (long long pa = 0; pa < s->nact; pa++) {
+        long long pg = pa + (long long)s->zlow * XY;
+        int c = s->ct[pg];
+        if (s->lid[pg] <= layer && c != SOLID && cycle > s->cts[pg]) {
+            s->U[pg] += s->dU[pg] * (float)((c == LIQUID) + (c == ACTIVE));
+            if (c == ACTIVE) s->steer[__atomic_fetch_add(&nsteer, 1, __ATOMIC_RELAXED)] = (int32_t)pa;
+        }
+    }
+    s->sum_front += (uint64_t)nsteer;
+#pragma omp parallel for num_threads(s->nthreads) schedule(static) reduction(+ : ncap, nsol)
+    for (long long q = 0; q < nsteer; q++) {
+        long long pa = s->steer[q];
+        int rz = (int)(pa / XY);
+        long long rem = pa % XY;
+        int rx = (int)(rem / Y), ry = (int)(rem % Y), gz = rz + s->zlow;
+        long long pg = ((long long)gz * X + rx) * Y + ry;
+        s->DL[pa] = (float)((double)s->DL[pa] + oracle_growth_increment(s->A, s->B, s->C, s->D, s->U[pg]));
+        int lcount = 0;
+        int b = it_bounds(rx, ry, X, Y);
+        for (int ll = 0; ll < list_len(b); ll++) {
+            int l = s->itlist[b][ll];
+            int mx = rx + s->nbx[l], my = ry + s->nby[l], mz = rz + s->nbz[l];
+            if (mz >= s->nzA) continue;
+            if (mz < 0) { /* below the window: never liquid in a well-formed run */
+                if (s->ct[((long long)(mz + s->zlow) * X + mx) * Y + my] == LIQUID) s->outside++;
+                continue;
+            }
+            long long na = ((long long)mz * X + mx) * Y + my;
+            long long ng = ((long long)(mz + s->zlow) * X + mx) * Y + my;
+            const int liquid = __atomic_load_n(&s->ct[ng], __ATOMIC_RELAXED) == LIQUID;
+            if (liquid) lcount = 1;
+            if (!(s->DL[pa] >= s->CDL[26 * pa + l] && liquid)) continue;
+            int32_t expected = LIQUID;
+            if (!__atomic_compare_exchange_n(&s->ct[ng], &expected, TEMP_UPDATE, 0, __ATOMIC_ACQ_REL,
+                                             __ATOMIC_RELAXED))
+                continue;
+            int h = s->gid[pg];
+            int o = orientation_of(s, h);
+            s->gid[ng] = h;
+            double cxold = s->DOC[3 * pa], cyold = s->DOC[3 * pa + 1], czold = s->DOC[3 * pa + 2];
+            double xp = (rx + s->xoff + s->nbx[l]) + 0.5, yp = (ry + s->yoff + s->nby[l]) + 0.5,
+                   zp = (gz + s->nbz[l]) + 0.5;
+            double c[3];
+            float dl;
+            capture_octahedron(&s->guv[9 * o], cxold, cyold, czold, xp, yp, zp, s->nbx, s->nby, s->nbz, c, &dl,
+                               &s->CDL[26 * na]);
+            s->DL[na] = dl;
+            s->DOC[3 * na] = (float)c[0];
+            s->DOC[3 * na + 1] = (float)c[1];
+            s->DOC[3 * na + 2] = (float)c[2];
+            __atomic_store_n(&s->ct[ng], ACTIVE, __ATOMIC_RELEASE);
+            ncap++;
+        }
+        if (lcount == 0) {
+            __atomic_store_n(&s->ct[pg], SOLID, __ATOMIC_RELAXED);
+            nsol++;
+        }
+    }
+    s->captures += ncap;
+    s->solidifications += nsol;
+}
+
+void oracle_set_threads(oracle_sim *s, int nthreads) { s->nthreads = nthreads; }
+
+static void capture_step(oracle_sim *s, int cycle, int layer) {
+    if (s->nthreads > 1)
+        cell_capture_omp(s, cycle, layer);
+    else
+        cell_capture(s, cycle, layer);
+}
+
 void oracle_step(oracle_sim *s, int cycle) {
-    nucleation(s, cycle, 0);
-    cell_capture(s, cycle, 0);
+    nucleation(s, cycle, s->layer);
+    capture_step(s, cycle, s->layer);
     s->cycle = cycle;
 }
 
+/* The transition to the next layer (src/runCA.cpp:343-418): DomainShiftAndResize
+ * (src/CAinitialize.cpp:2772-2832) moves the active window's bottom to just below the lowest
+ * active cell and its top to the highest cell of the next layer; LayerSetup (:2835-2982) zeroes
+ * the window's octahedron arrays and gives every active cell in it a cell-centred octahedron
+ * (half-diagonal 0.01, critical lengths of its orientation). The cycle count restarts at 0. */
+int oracle_next_layer(oracle_sim *s) {
+    if (s->layer + 1 >= s->nlayers) return fail("no layer after layer %d", s->layer);
+    const long long XY = (long long)s->X * s->Y;
+    int newmin = 0x7FFFFFFF, newmax = (int)0x80000000;
+    for (long long p = 0; p < s->ndom; p++) {
+        const int z = (int)(p / XY);
+        if (s->ct[p] == ACTIVE && z < newmin) newmin = z;
+        if (s->lid[p] == s->layer + 1 && z > newmax) newmax = z;
+    }
+    s->zlow = newmin - 1;
+    s->zhigh = newmax;
+    s->nzA = s->zhigh - s->zlow + 1;
+    if (s->zlow < 0 || s->nzA < 1 || s->zhigh >= s->nz) return fail("no active window for layer %d", s->layer + 1);
+    s->nact = XY * s->nzA;
+    free(s->DL);
+    free(s->DOC);
+    free(s->CDL);
+    free(s->steer);
+    s->DL = calloc((size_t)s->nact, 4);
+    s->DOC = calloc((size_t)s->nact * 3, 4);
+    s->CDL = calloc((size_t)s->nact * 26, 4);
+    s->steer = calloc((size_t)s->nact, 4);
+    if (!s->DL || !s->DOC || !s->CDL || !s->steer) return fail("out of host memory");
+    for (long long pa = 0; pa < s->nact; pa++) {
+        const long long pg = pa + (long long)s->zlow * XY;
+        if (s->ct[pg] != ACTIVE) continue;
+        const int rz = (int)(pa / XY);
+        const long long rem = pa % XY;
+        const int rx = (int)(rem / s->Y), ry = (int)(rem % s->Y);
+        const int o = orientation_of(s, s->gid[pg]);
+        s->DL[pa] = 0.01f;
+        s->DOC[3 * pa] = (float)(rx + s->xoff + 0.5);
+        s->DOC[3 * pa + 1] = (float)(ry + s->yoff + 0.5);
+        s->DOC[3 * pa + 2] = (float)(rz + s->zlow + 0.5);
+        seed_cdl(&s->guv[9 * o], s->nbx, s->nby, s->nbz, &s->CDL[26 * pa]);
+    }
+    s->layer++;
+    s->cycle = 0;
+    s->nn_layer = s->nn;
+    return 0;
+}
+
 void oracle_check(oracle_sim *s, int *cycle, uint64_t counts[5], int *xswitch, int verbose) {
-    const int layer = 0;
+    const int layer = s->layer;
     uint64_t c[5] = {0, 0, 0, 0, 0};
     for (long long p = 0; p < s->ndom; p++) {
         if (s->lid[p] == layer) {
@@ -1186,7 +1448,7 @@ void oracle_check(oracle_sim *s, int *cycle, uint64_t counts[5], int *xswitch, i
     if (verbose)
         printf("cycle = %d Superheated liquid cells = %lu Undercooled liquid cells = %lu Number of nucleation "
                "events this layer %lld Remaining liquid cells in future layers of this simulation = %lu\n",
-               *cycle, (unsigned long)c[0], (unsigned long)c[1], s->nn, (unsigned long)c[4]);
+               *cycle, (unsigned long)c[0], (unsigned long)c[1], s->nn - s->nn_layer, (unsigned long)c[4]);
     if (verbose) fflush(stdout);
     if (c[0] + c[1] == 0) *xswitch = 1;
     if (*xswitch == 0 && strcmp(s->type, "S") == 0 && c[1] == 0) {
@@ -1209,12 +1471,12 @@ void oracle_check(oracle_sim *s, int *cycle, uint64_t counts[5], int *xswitch, i
 int oracle_run(oracle_sim *s, int max_cycles, int verbose, double *capture_seconds) {
     int cycle = s->cycle, xswitch = 0;
     double tcap = 0;
-    do {
+    for (;;) {
         cycle++;
-        nucleation(s, cycle, 0);
+        nucleation(s, cycle, s->layer);
         struct timespec t0, t1;
         clock_gettime(CLOCK_MONOTONIC, &t0);
-        cell_capture(s, cycle, 0);
+        capture_step(s, cycle, s->layer);
         clock_gettime(CLOCK_MONOTONIC, &t1);
         tcap += (t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec - t0.tv_nsec);
         s->cycle = cycle;
@@ -1224,7 +1486,15 @@ int oracle_run(oracle_sim *s, int max_cycles, int verbose, double *capture_secon
             s->cycle = cycle;
         }
         if (max_cycles > 0 && cycle >= max_cycles) break;
-    } while (xswitch == 0);
+        if (xswitch) {
+            /* a whole run goes on through every layer (src/runCA.cpp:283-418) */
+            if (max_cycles != 0 || s->layer + 1 >= s->nlayers) break;
+            if (oracle_next_layer(s)) break;
+            if (verbose) printf("New layer %d, active window z = %d .. %d\n", s->layer, s->zlow, s->zhigh);
+            cycle = 0;
+            xswitch = 0;
+        }
+    }
     if (capture_seconds) *capture_seconds = tcap;
     return cycle;
 }

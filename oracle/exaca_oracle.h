@@ -37,9 +37,15 @@ enum {
 
 /* Parse deck + initialise (np = 1). Returns 0 on success; message via oracle_last_error(). */
 int oracle_load(const char *deck_path, const char *data_dir, oracle_sim **out);
+/* The same with an ensemble instance's random streams (type C): mt19937_64(seed) for the substrate
+ * and the nucleation sites, default_random_engine(seed % 2147483646 + 1) for the nucleation
+ * undercoolings; seed 0 is oracle_load (the reference's fixed streams). */
+int oracle_load_seeded(const char *deck_path, const char *data_dir, uint64_t seed, oracle_sim **out);
 /* dims: [0]=X(MyXSlices) [1]=Y(MyYSlices) [2]=nz [3]=nzActive [4]=nx [5]=ny [6]=P (nuclei)
- *       [7]=cycle [8]=nn [9]=sum of frontier sizes [10]=captures [11]=solidifications */
-void oracle_dims(const oracle_sim *s, int64_t dims[12]);
+ *       [7]=cycle [8]=nn [9]=sum of frontier sizes [10]=captures [11]=solidifications
+ *       [12]=layers [13]=current layer [14]=ZBound_Low [15]=ZBound_High
+ *       [16]=nuclei or captures that fell outside the active window (0 in a well-formed run) */
+void oracle_dims(const oracle_sim *s, int64_t dims[17]);
 /* Scalar parameters: [0]=A [1]=B [2]=C (normalised) [3]=D [4]=deltax [5]=deltat [6]=G [7]=R
  *                    [8]=UndercoolingChange of the first liquid cell */
 void oracle_params(const oracle_sim *s, double out[9]);
@@ -48,9 +54,15 @@ int oracle_get_field(const oracle_sim *s, int which, void *dst);
 void oracle_step(oracle_sim *s, int cycle);
 /* IntermediateOutputAndCheck (src/CAupdate.cpp:732-822); counts[5] as in the reference. */
 void oracle_check(oracle_sim *s, int *cycle, uint64_t counts[5], int *xswitch, int verbose);
-/* The whole layer loop (src/runCA.cpp:283-342); stops early after max_cycles (<=0: no cap).
- * Returns the final cycle. capture_seconds receives the CellCapture wall time. */
+/* The layer loop (src/runCA.cpp:283-342) from the current cycle; with max_cycles == 0 it goes on
+ * through every remaining layer (the transitions of src/runCA.cpp:343-418), with max_cycles < 0 it
+ * ends with the current layer, otherwise it stops after max_cycles in the current layer. Returns
+ * the final cycle (of the last layer run).
+ * capture_seconds receives the CellCapture wall time. */
 int oracle_run(oracle_sim *s, int max_cycles, int verbose, double *capture_seconds);
+/* DomainShiftAndResize + LayerSetup (src/CAinitialize.cpp:2772-2982) and the cycle reset: the next
+ * layer's active window and cell-centred octahedra for its active cells. */
+int oracle_next_layer(oracle_sim *s);
 /* vtk + misorientation vtk + log, formats of src/CAprint.cpp:285-571 (timings written as 0). */
 int oracle_write_outputs(const oracle_sim *s, const char *deck_path);
 void oracle_free(oracle_sim *s);
@@ -70,6 +82,19 @@ void oracle_capture_octahedron(const float guv[9], double cxold, double cyold, d
                                double yp, double zp, double centre[3], float *diag, float cdl[26]);
 /* Growth increment min(0.045, max(0, V(min(210, U)))) (src/CAupdate.cpp:257-261). */
 double oracle_growth_increment(double A, double B, double C, double D, float U);
+
+/* A simulation of a type-C deck started from another run's state at the end of step `cycle`
+ * (the MI355X path's download): ct_raw (cell type | 0x10 for cell-centred octahedra, uint8 per
+ * cell), GrainID, DiagonalLength (float) and octahedron centres (double[3]) per cell, and the P
+ * nucleation sites. The temperature fields are the oracle's own; UndercoolingCurrent and the
+ * critical lengths are derived as the reference accumulates and computes them. */
+int oracle_load_state(const char *deck, const char *data_dir, int cycle, const uint8_t *ct_raw, const int32_t *gid,
+                      const float *dl, const double *centre, int P, const int32_t *nloc, const int32_t *ntime,
+                      oracle_sim **out);
+/* nthreads > 1: CellCapture runs in parallel as the reference's OpenMP backend does (atomic
+ * frontier appends, compare-exchange captures): a timing baseline whose results depend on thread
+ * timing. 0 or 1: the serial restatement. */
+void oracle_set_threads(oracle_sim *s, int nthreads);
 
 /* ---- test hooks: hand-built states for the capture-order pins ----
  * oracle_set_field overwrites a whole field (the layouts of oracle_get_field; not the nucleus

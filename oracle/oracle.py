@@ -30,6 +30,8 @@ def lib() -> ctypes.CDLL:
         vp, i64p = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)
         L.oracle_load.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(vp)]
         L.oracle_load.restype = ctypes.c_int
+        L.oracle_load_seeded.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint64, ctypes.POINTER(vp)]
+        L.oracle_load_seeded.restype = ctypes.c_int
         L.oracle_dims.argtypes = [vp, i64p]
         L.oracle_params.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
         L.oracle_get_field.argtypes = [vp, ctypes.c_int, vp]
@@ -42,6 +44,8 @@ def lib() -> ctypes.CDLL:
         L.oracle_write_outputs.argtypes = [vp, ctypes.c_char_p]
         L.oracle_write_outputs.restype = ctypes.c_int
         L.oracle_free.argtypes = [vp]
+        L.oracle_next_layer.argtypes = [vp]
+        L.oracle_next_layer.restype = ctypes.c_int
         L.oracle_last_error.restype = ctypes.c_char_p
         L.oracle_mt64_draws.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]
         L.oracle_uniform_draws.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
@@ -58,6 +62,10 @@ def lib() -> ctypes.CDLL:
         L.oracle_set_orientation.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
         L.oracle_set_orientation.restype = ctypes.c_int
         L.oracle_set_mutation.argtypes = [vp, ctypes.c_int]
+        L.oracle_set_threads.argtypes = [vp, ctypes.c_int]
+        L.oracle_load_state.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, vp, vp, vp, vp, ctypes.c_int,
+                                        vp, vp, ctypes.POINTER(vp)]
+        L.oracle_load_state.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -69,11 +77,35 @@ def _ptr(a, t):
 class OracleSim:
     """One deck, initialised and stepped serially on the CPU (reference Serial semantics)."""
 
-    def __init__(self, deck, data_dir):
+    def __init__(self, deck, data_dir, seed: int = 0):
         self._h = ctypes.c_void_p()
-        if lib().oracle_load(str(deck).encode(), str(data_dir).encode(), ctypes.byref(self._h)) != 0:
+        if lib().oracle_load_seeded(str(deck).encode(), str(data_dir).encode(), seed, ctypes.byref(self._h)) != 0:
             raise RuntimeError("oracle: " + lib().oracle_last_error().decode())
         self.deck = str(deck)
+
+    @classmethod
+    def from_state(cls, deck, data_dir, cycle: int, ct_raw, gid, dl, centre, nloc, ntime) -> "OracleSim":
+        """A type-C simulation started from another run's state at the end of step `cycle`:
+        ct_raw (uint8: type | 0x10 for cell-centred octahedra), GrainID (int32), DiagonalLength
+        (float32) and centres (float64, 3 per cell) of every cell, and the nucleation sites."""
+        self = cls.__new__(cls)
+        self._h = ctypes.c_void_p()
+        arrs = [np.ascontiguousarray(ct_raw, dtype=np.uint8), np.ascontiguousarray(gid, dtype=np.int32),
+                np.ascontiguousarray(dl, dtype=np.float32), np.ascontiguousarray(centre, dtype=np.float64),
+                np.ascontiguousarray(nloc, dtype=np.int32), np.ascontiguousarray(ntime, dtype=np.int32)]
+        if not (arrs[0].size == arrs[1].size == arrs[2].size == arrs[3].size // 3 and arrs[4].size == arrs[5].size):
+            raise ValueError("inconsistent state arrays")
+        ct, g, d, c, nl, nt = arrs
+        if lib().oracle_load_state(str(deck).encode(), str(data_dir).encode(), cycle, ct.ctypes.data, g.ctypes.data,
+                                   d.ctypes.data, c.ctypes.data, nl.size, nl.ctypes.data, nt.ctypes.data,
+                                   ctypes.byref(self._h)) != 0:
+            raise RuntimeError("oracle: " + lib().oracle_last_error().decode())
+        self.deck = str(deck)
+        return self
+
+    def set_threads(self, n: int) -> None:
+        """n > 1: CellCapture as the reference's OpenMP backend runs it (timing only)."""
+        lib().oracle_set_threads(self._h, n)
 
     def close(self):
         if self._h:
@@ -84,9 +116,10 @@ class OracleSim:
         self.close()
 
     def dims(self) -> dict:
-        d = (ctypes.c_int64 * 12)()
+        d = (ctypes.c_int64 * 17)()
         lib().oracle_dims(self._h, d)
-        keys = ["X", "Y", "nz", "nzA", "nx", "ny", "nuclei", "cycle", "nn", "frontier", "captures", "solidified"]
+        keys = ["X", "Y", "nz", "nzA", "nx", "ny", "nuclei", "cycle", "nn", "frontier", "captures", "solidified",
+                "layers", "layer", "zlow", "zhigh", "outside"]
         return dict(zip(keys, list(d)))
 
     def params(self) -> dict:
@@ -138,6 +171,11 @@ class OracleSim:
         t = ctypes.c_double(0)
         cyc = lib().oracle_run(self._h, max_cycles, int(verbose), ctypes.byref(t))
         return cyc, t.value
+
+    def next_layer(self) -> None:
+        """DomainShiftAndResize + LayerSetup after the current layer."""
+        if lib().oracle_next_layer(self._h) != 0:
+            raise RuntimeError(lib().oracle_last_error().decode())
 
     def write_outputs(self) -> None:
         if lib().oracle_write_outputs(self._h, self.deck.encode()) != 0:

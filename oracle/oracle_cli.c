@@ -17,7 +17,7 @@ int main(int argc, char **argv) {
     }
     double tcap = 0;
     int cyc = oracle_run(s, argc > 3 ? atoi(argv[3]) : 0, 1, &tcap);
-    int64_t d[12];
+    int64_t d[17];
     oracle_dims(s, d);
     printf("final cycle %d nn %lld frontier-sum %lld captures %lld solidified %lld capture-time %.3f s\n", cyc,
            (long long)d[8], (long long)d[9], (long long)d[10], (long long)d[11], tcap);

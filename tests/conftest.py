@@ -50,7 +50,7 @@ Debug check (extensive) - Print data for all main data structures (CellType, Gra
 
 
 def s_deck(path: Path, *, nsx=2, nsy=2, offset=30, radius=20, density=10.0, G=1e6, R=1e6, N=10, spacing=25,
-           material="Inconel625_Baseline", base="spots", out="./", printing="N") -> Path:
+           layers=1, layer_height=20, material="Inconel625_Baseline", base="spots", out="./", printing="N") -> Path:
     """Writes a spot-melt array (type S) deck in the reference's format."""
     path.write_text(f"""small spot melt array test case
 *****
@@ -70,8 +70,8 @@ Number of spots in x: {nsx}
 Number of spots in y: {nsy}
 Offset between spot centers (in microns): {offset}
 Radii of spots (in microns): {radius}
-Number of layers: 1
-Offset between layers (in cells): 20
+Number of layers: {layers}
+Offset between layers (in cells): {layer_height}
 "N", where the time step = deltax/(Steady State Velocity * N): {N}
 Substrate grain spacing (in microns): {spacing}
 ***Output data printing options: (Y or N) which data should be printed***

@@ -126,3 +126,50 @@ def test_admission_respects_the_memory_budget(tmp_path):
     with ea.Ensemble(slots=8, arena_bytes=need + need // 2) as ens:
         assert ens.admit(preps[0]) is not None
         assert ens.admit(preps[0]) is None  # no room for a second copy
+
+
+@pytest.mark.parametrize("k", range(1, len(CASES)))
+def test_seeded_instance_matches_the_oracle(tmp_path, k):
+    """An ensemble instance's own random streams (mt19937_64(seed) for the substrate and the sites,
+    default_random_engine(seed % 2147483646 + 1) for the undercoolings): the plain path with that
+    seed equals the oracle initialised with the same streams, cell for cell."""
+    deck, seed = _decks(tmp_path)[k]
+    assert seed != 0
+    cycle, ct, gid, q = _alone(deck, seed)
+    o = orc.OracleSim(deck, DATA, seed=seed)
+    ocycle, _ = o.run()
+    d = o.dims()
+    assert cycle == ocycle
+    assert (q["nn"], q["captures"], q["solidified"], q["frontier"]) == (d["nn"], d["captures"], d["solidified"],
+                                                                      d["frontier"])
+    assert np.array_equal(ct, o.field(orc.CELLTYPE)) and np.array_equal(gid, o.field(orc.GRAINID))
+
+
+def test_process_map_ensemble_and_serving_256(tmp_path):
+    """The batched-ensemble and serving workloads in full: 256 process-map requests (sides
+    log-uniform in [32, 256], G, R and nucleation density log-uniform, one seed each). Solved all
+    at once (every instance resident together, the autotuner choosing workgroups per slot on the
+    live mix), served as they arrive (Poisson 50 req/s, continuous batching under a 200 GiB
+    budget) and one after another on the plain path: every request's digest (final cycle,
+    CellType, GrainID) must be the same three times. The four smallest instances are also run on
+    the oracle with their own seeds."""
+    from exaca_amd import decks, serve
+
+    reqs = decks.process_map_requests(256, tmp_path, rate=50.0)
+    print("[ensemble] 256 requests, batched", flush=True)
+    batched = serve.run_served(reqs, realtime=False)
+    assert batched["peak_resident"] == 256 and not batched["failed"]
+    assert batched["blocks_per_slot"] in (16, 32, 64, 128, 256)
+    print("[ensemble] served, Poisson 50 req/s", flush=True)
+    served = serve.run_served(reqs, realtime=True)
+    assert not served["failed"]
+    print("[ensemble] sequential plain path", flush=True)
+    seq = serve.run_sequential(reqs)
+    for r in reqs:
+        assert batched["digests"][r.index] == seq["digests"][r.index] == served["digests"][r.index], r.index
+    for r in sorted(reqs, key=lambda r: r.nx * r.ny * r.nz)[:4]:
+        print(f"[ensemble] oracle check of request {r.index} ({r.nx}x{r.ny}x{r.nz}, seed {r.seed})", flush=True)
+        o = orc.OracleSim(r.deck, DATA, seed=r.seed)
+        ocycle, _ = o.run()
+        assert serve.digest(ocycle, o.field(orc.CELLTYPE), o.field(orc.GRAINID)) == seq["digests"][r.index]
+        o.close()

@@ -63,3 +63,49 @@ def test_800_full_size_conservation_and_determinism():
     for key in ("nn", "captures", "solidified", "frontier"):
         assert q2[key] == q[key], key
     assert (hashlib.sha1(ct2.tobytes()).hexdigest(), hashlib.sha1(gid2.tobytes()).hexdigest()) == digest
+
+
+# The oracle cannot step the 800^3 deck from its start in test time, but it can continue the GPU's
+# own state: at cycle 6000 (the middle of the run, a fully developed front of 3.4 M active cells)
+# the GPU's cell types, GrainIDs, half-diagonals and double octahedron centres are imported into
+# the oracle (which derives the undercooling and the critical lengths as the reference holds them)
+# and both paths step 12 more cycles. Every cell's CellType and GrainID, and the active cells'
+# DiagonalLength and DOCenter, must then agree bit for bit: the late-front regime at full size.
+import oracle as orc  # noqa: E402
+from conftest import DATA  # noqa: E402
+
+START, MORE = 6000, 12
+
+
+def test_800_front_continues_exactly_on_the_oracle():
+    print(f"[800^3] GPU to cycle {START}", flush=True)
+    with ea.Solver(DECK) as sol:
+        sol.run(max_cycles=START)
+        st = sol.front_state()
+        print(f"[800^3] GPU {MORE} more steps", flush=True)
+        sol.run(max_cycles=START + MORE)
+        q = sol.query()
+        ct_g = sol.field(ea.FIELD_CELLTYPE)
+        gid_g = sol.field(ea.FIELD_GRAINID)
+        dl_g = sol.field(ea.FIELD_DIAGONAL)
+        doc_g = sol.field(ea.FIELD_DOCENTER).reshape(-1, 3)
+    active0 = int(np.count_nonzero((st["ct_raw"] & 0x0F) == ACTIVE))
+    assert active0 > 1_000_000
+    print(f"[800^3] oracle import of {active0} active cells", flush=True)
+    o = orc.OracleSim.from_state(DECK, DATA, START, st["ct_raw"], st["gid"], st["dl"], st["centre"], st["nloc"],
+                                 st["ntime"])
+    del st
+    print(f"[800^3] oracle {MORE} steps", flush=True)
+    cycle, _ = o.run(max_cycles=START + MORE)
+    assert cycle == START + MORE
+    d = o.dims()
+    ct_o = o.field(orc.CELLTYPE)
+    assert np.array_equal(ct_g, ct_o), f"CellType differs at {np.count_nonzero(ct_g != ct_o)} cells"
+    assert np.array_equal(gid_g, o.field(orc.GRAINID)), "GrainID differs"
+    act = np.flatnonzero(ct_o == ACTIVE)
+    assert np.array_equal(dl_g[act].view(np.uint32), o.field(orc.DIAGLEN)[act].view(np.uint32))
+    assert np.array_equal(doc_g[act].view(np.uint32), o.field(orc.DOCENTER).reshape(-1, 3)[act].view(np.uint32))
+    assert d["captures"] > 500_000, "the window should hold the steady front's captures"
+    print(f"[800^3] {d['captures']} captures, {d['solidified']} solidifications, {d['frontier']} active-cell updates "
+          f"identical", flush=True)
+    o.close()

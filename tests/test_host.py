@@ -28,6 +28,10 @@ DECKS = {
     # block generator and integer cut must reproduce <random>'s decisions draw for draw
     "dense_nuclei": lambda p: c_deck(p / "n.txt", 21, nx=19, ny=23, density=300000, fraction=0.1),
     "all_nuclei": lambda p: c_deck(p / "a.txt", 8, density=2000000, fraction=0.1),
+    # multilayer spot melts: later layers overwrite CritTimeStep and LayerID where their pools overlap
+    "spot_melt_2_layers": lambda p: EXAMPLES / "Inp_SpotMelt.txt",
+    "spots_3_layers": lambda p: s_deck(p / "l.txt", nsx=2, nsy=2, offset=16, radius=10, density=300, spacing=8,
+                                       layers=3, layer_height=6),
 }
 
 
@@ -90,9 +94,41 @@ def test_deck_errors(edit, msg, tmp_path):
         orc.OracleSim(deck, DATA)
 
 
-def test_multilayer_rejected():
-    with pytest.raises(ea.ExacaError, match="multilayer"):
-        ea.host_state(EXAMPLES / "Inp_SpotMelt.txt")
+def test_multilayer_deck_values():
+    """Inp_SpotMelt.txt: 2 layers 20 cells apart; nz = r + 3 + (layers - 1) * offset, and the first
+    layer's active window is r + 2 planes."""
+    d = ea.host_state(EXAMPLES / "Inp_SpotMelt.txt")
+    assert (d["nz"], d["nzA"]) == (75 + 3 + 20, 77)
+
+
+def test_oracle_multilayer_run(tmp_path):
+    """The oracle through three layers: each transition moves the window to just below the lowest
+    active cell and up to the next layer's top (src/CAinitialize.cpp:2772-2832), the cycle count
+    restarts, no nucleus or capture falls outside a window, and each layer ends with none of its
+    cells liquid."""
+    deck = s_deck(tmp_path / "l.txt", nsx=2, nsy=2, offset=16, radius=10, density=300, spacing=8, layers=3,
+                  layer_height=6)
+    o = orc.OracleSim(deck, DATA)
+    lid = o.field(orc.LAYERID)
+    d = o.dims()
+    assert d["layers"] == 3 and (d["zlow"], d["zhigh"]) == (0, 11)
+    XY = d["X"] * d["Y"]
+    tops = [int(np.flatnonzero(lid == k).max() // XY) for k in range(3)]
+    assert tops == [11, 17, 23]
+    for layer in range(3):
+        o.run(max_cycles=-1)  # this layer to its end
+        d = o.dims()
+        assert d["layer"] == layer and d["outside"] == 0
+        ct = o.field(orc.CELLTYPE)
+        assert not np.any((ct == 5) & (lid == layer)), "liquid left in the finished layer"
+        if layer < 2:
+            lowest = int((np.flatnonzero(ct == 2) // XY).min())
+            o.next_layer()
+            d = o.dims()
+            assert (d["zlow"], d["zhigh"], d["cycle"]) == (lowest - 1, tops[layer + 1], 0)
+    assert not np.any(o.field(orc.CELLTYPE) == 5)
+    with pytest.raises(RuntimeError, match="no layer after"):
+        o.next_layer()
 
 
 def test_library_exports_every_declared_symbol():

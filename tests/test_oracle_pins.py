@@ -369,3 +369,57 @@ def test_last_liquid_neighbour_taken_lower_or_higher(tmp_path, capturer_below):
     o2.set_mutation(orc.MUT_DESCENDING)
     o2.step(1)
     assert o2.field(orc.CELLTYPE)[c] == (ACTIVE if capturer_below else SOLID)
+
+
+# ---------------------------------------------------------------- state import and the OpenMP pass
+def _own_state(o):
+    """The oracle's state at a cycle before any capture: every active cell holds a cell-centred
+    octahedron (initial seed or nucleus), so its centre is the cell centre."""
+    d = o.dims()
+    X, Y, nz = d["X"], d["Y"], d["nz"]
+    ct = o.field(orc.CELLTYPE).astype(np.uint8)
+    k, i, j = np.meshgrid(np.arange(nz), np.arange(X), np.arange(Y), indexing="ij")
+    centre = np.stack([i - 1 + 0.5, j - 1 + 0.5, k + 0.5], axis=-1).reshape(-1).astype(np.float64)
+    raw = ct | np.where(ct == ACTIVE, 0x10, 0).astype(np.uint8)
+    dl = np.zeros(ct.size, dtype=np.float32)
+    dl[:X * Y * d["nzA"]] = o.field(orc.DIAGLEN)
+    return raw, o.field(orc.GRAINID), dl, centre, o.field(orc.NUCLEI_LOC), o.field(orc.NUCLEI_TIME)
+
+
+def test_state_import_continues_the_run(tmp_path):
+    """A run imported at cycle 200 (undercooling accumulated over up to 210 steps, nuclei placed,
+    no capture yet: the first comes at cycle 232) continues exactly as the run it was taken from."""
+    deck = c_deck(tmp_path / "i.txt", 24, density=30000, fraction=0.05)
+    a = orc.OracleSim(deck, DATA)
+    a.run(max_cycles=200)
+    assert a.dims()["captures"] == 0 and a.dims()["nn"] > 100
+    b = orc.OracleSim.from_state(deck, DATA, 200, *_own_state(a))
+    assert np.array_equal(b.field(orc.UNDERCOOLING), a.field(orc.UNDERCOOLING) * (
+        np.isin(a.field(orc.CELLTYPE), [ACTIVE, LIQUID])))
+    assert np.array_equal(b.field(orc.CRITDIAG), a.field(orc.CRITDIAG) * np.repeat(
+        a.field(orc.CELLTYPE)[:b.field(orc.DIAGLEN).size] == ACTIVE, 26))
+    for cycle in range(201, 600):
+        a.step(cycle)
+        b.step(cycle)
+    assert a.dims()["captures"] > 1000
+    for f in (orc.CELLTYPE, orc.GRAINID, orc.DIAGLEN, orc.DOCENTER):
+        assert np.array_equal(a.field(f), b.field(f)), f
+
+
+def test_openmp_pass_conserves_cells(tmp_path):
+    """The OpenMP CellCapture (a timing baseline: which cell wins a contested capture depends on
+    thread timing) still accounts for every cell: each solidification turns one active cell solid,
+    each capture or nucleation event one liquid cell active, and the run ends with no liquid."""
+    deck = c_deck(tmp_path / "p.txt", 30, density=300, fraction=0.05)
+    o = orc.OracleSim(deck, DATA)
+    o.set_threads(4)
+    ct0 = o.field(orc.CELLTYPE)
+    o.run()
+    d = o.dims()
+    ct = o.field(orc.CELLTYPE)
+    assert int((ct == LIQUID).sum()) == 0
+    assert int((ct == SOLID).sum()) - int((ct0 == SOLID).sum()) == d["solidified"]
+    # nucleation events: at least the interior ones (nn), at most one per site
+    events = int((ct == ACTIVE).sum()) - int((ct0 == ACTIVE).sum()) - d["captures"] + d["solidified"]
+    assert d["nn"] <= events <= d["nuclei"]
+    assert int((ct0 == LIQUID).sum()) == d["captures"] + events
