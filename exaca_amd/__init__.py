@@ -76,6 +76,8 @@ _SIGNATURES = {
     "exaca_ensemble_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "exaca_grain_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32), _I64P,
                                          ctypes.POINTER(ctypes.c_double), _I64P]),
+    "exaca_grain_stats_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32), _I64P,
+                                                 ctypes.POINTER(ctypes.c_double), _I64P]),
     "exaca_next_layer": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
 }
 
@@ -343,10 +345,11 @@ class Solver:
                 "dl": self.field(FIELD_DIAGONAL), "centre": self.field(FIELD_CENTRE_F64),
                 "nloc": self.field(FIELD_NUCLEI_LOC), "ntime": self.field(FIELD_NUCLEI_TIME)}
 
-    def grain_stats(self, region=None) -> dict:
+    def grain_stats(self, region=None, noalloc: bool = False) -> dict:
         """Grain statistics of a region (XMin, XMax, YMin, YMax, ZMin, ZMax, inclusive, in the
         coordinates of the reference's vtk files; default: every cell inside the walls), computed on
-        the GPU from the current GrainID field (exaca_grain_stats)."""
+        the GPU from the current GrainID field (exaca_grain_stats, or exaca_grain_stats_noalloc,
+        which allocates nothing)."""
         q = self.query()
         if region is None:
             region = (1, q["nx"] - 2, 1, q["ny"] - 2, 1, q["nz"] - 2)
@@ -354,7 +357,8 @@ class Solver:
         counts = (ctypes.c_int64 * 5)()
         values = (ctypes.c_double * 4)()
         hist = (ctypes.c_int64 * 64)()
-        self._ok(self._lib.exaca_grain_stats(self._h, reg, counts, values, hist))
+        fn = self._lib.exaca_grain_stats_noalloc if noalloc else self._lib.exaca_grain_stats
+        self._ok(fn(self._h, reg, counts, values, hist))
         return {"cells": counts[0], "melted": counts[1], "grains": counts[2], "nucleated_cells": counts[3],
                 "melted_top": counts[4], "mean_misorientation": values[0], "mean_misorientation_top": values[1],
                 "mean_grain_volume": values[2], "nucleated_fraction": values[3], "histogram": list(hist)}

@@ -8,6 +8,7 @@
 #include <chrono>
 #include <cstdlib>
 #include <memory>
+#include <mutex>
 #include <cstdio>
 #include <cstring>
 #include <iostream>
@@ -618,6 +619,51 @@ void grain_stats_finish(const exaca_state *h, const int32_t region[6], const uns
     if (hist) std::copy(hb, hb + 64, hist);
 }
 
+// ---- The static scratch of exaca_grain_stats_noalloc: counters, orientation bins and GrainID
+// flags in one file-scope device array (each GPU holds its own copy until the process ends). ----
+constexpr size_t kScratchCountsOff = 0;
+constexpr size_t kScratchBinsOff = 256;
+constexpr size_t kScratchFlagsOff = kScratchBinsOff + ((8ull * exaca::kNumOrientations + 255) / 256) * 256;
+constexpr uint32_t kScratchMaxGrainIds = 1u << 20;
+constexpr size_t kScratchBytes = kScratchFlagsOff + 4ull * kScratchMaxGrainIds;
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+// Device address of `scratch` on the current device, looked up once per device (host code must
+// never pass the symbol itself to a kernel: on the host it names a stand-in, not device memory).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void *g_scratch_ptr[kMaxDevices] = {};
+void *scratch_address() {
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) throw DeviceError("device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
+}
+
+// Makes the handle's device current for the duration of a call.
+struct DeviceScope {
+    int prev = -1;
+    explicit DeviceScope(int dev) {
+        HIP_OK(hipGetDevice(&prev));
+        if (prev != dev) HIP_OK(hipSetDevice(dev));
+        else prev = -1;
+    }
+    ~DeviceScope() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
 }  // namespace
 
 extern "C" {
@@ -729,6 +775,35 @@ int exaca_grain_stats(exaca_state *h, const int32_t region[6], int64_t counts[5]
         HIP_OK(hipMemcpyAsync(c, b.cnt, sizeof c, hipMemcpyDeviceToHost, h->stream));
         HIP_OK(hipStreamSynchronize(h->stream));
         grain_stats_finish(h, region, c, ob.data(), counts, values, hist);
+    });
+}
+
+int exaca_grain_stats_noalloc(exaca_state *h, const int32_t region[6], int64_t counts[5], double values[4],
+                              int64_t hist[64]) {
+    return guarded([&] {
+        if (!h || !region) throw std::invalid_argument("null argument");
+        if (h->pending >= 0) throw StateError("grain_stats called between nucleation and capture");
+        const exaca::GrainRegion r = checked_region(h, region);
+        grain_id_range(h);
+        if (h->ngid > kScratchMaxGrainIds)
+            throw std::invalid_argument("exaca_grain_stats_noalloc: the domain's GrainID range holds " +
+                                        std::to_string(h->ngid) + " values, more than the " +
+                                        std::to_string(kScratchMaxGrainIds) +
+                                        " its static scratch covers; use exaca_grain_stats");
+        DeviceScope on(h->device);
+        char *base = static_cast<char *>(scratch_address());
+        StatsBuffers b{};
+        b.cnt = reinterpret_cast<unsigned long long *>(base + kScratchCountsOff);
+        b.bins = reinterpret_cast<unsigned int *>(base + kScratchBinsOff);
+        b.gflag = reinterpret_cast<uint32_t *>(base + kScratchFlagsOff);
+        grain_stats_pass(h, r, b);
+        // host copies into per-thread static arrays: this entry point allocates nothing
+        static thread_local unsigned int ob[2 * exaca::kNumOrientations];
+        static thread_local unsigned long long c[4];
+        HIP_OK(hipMemcpyAsync(ob, b.bins, sizeof ob, hipMemcpyDeviceToHost, h->stream));
+        HIP_OK(hipMemcpyAsync(c, b.cnt, sizeof c, hipMemcpyDeviceToHost, h->stream));
+        HIP_OK(hipStreamSynchronize(h->stream));
+        grain_stats_finish(h, region, c, ob, counts, values, hist);
     });
 }
 

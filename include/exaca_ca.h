@@ -106,6 +106,16 @@ int exaca_next_layer(exaca_state *h, int layer, int verbose);
  * whole degrees (the misorientation vtk's values). Any output pointer may be NULL. */
 int exaca_grain_stats(exaca_state *h, const int32_t region[6], int64_t counts[5], double values[4], int64_t hist[64]);
 
+/* exaca_grain_stats without any device or host allocation (no counterpart in the reference: for
+ * callers that cannot allocate, such as code under HIP graph capture or a serving loop that
+ * forbids it). Same arguments and results. Its counters, orientation bins and GrainID flags live
+ * in one static device array of 4.2 MB per GPU, held until the process ends; domains whose
+ * initial GrainID range spans more than 2^20 values are rejected with EXACA_ERR_INPUT. Every call
+ * on a GPU shares that GPU's array: calls that overlap on one GPU (several host threads, or
+ * handles on streams run concurrently) race and must be serialised by the caller. */
+int exaca_grain_stats_noalloc(exaca_state *h, const int32_t region[6], int64_t counts[5], double values[4],
+                              int64_t hist[64]);
+
 /* Releases the handle and its device memory. */
 int exaca_destroy(exaca_state *h);
 
