@@ -10,7 +10,7 @@ active-cell updates per second, i.e. the sum over the timed steps of the frontie
 their liquidus time, the reference's numSteer) divided by the device time of those steps (HIP
 events on the solver's stream).
 
-roofline: the dominant work is CellCapture (k_claim, k_apply, k_settle and k_rebuild, the
+roofline: the dominant work is CellCapture (k_claim, k_win, k_apply, k_settle and k_rebuild, the
 reference's CellCapture kernel). Its algorithmic bytes are 137 B per active-cell update + 125 B per
 capture + 1 B per solidification (the reference layout's bytes: own state, the 26 critical lengths
 and the new cell's octahedron), divided by the kernels' event-timed durations over the same cycles
@@ -277,7 +277,7 @@ def main() -> int:
             raise RuntimeError(f"event-timed replay of the window differs in {k}: {prof[k]} vs {q1[k] - q0[k]}")
     kms = prof["ms"]
     nsteps = max(prof["steps"], 1)
-    # CellCapture = claim + apply + settle + the rebuild of the next list (the reference's pass 1
+    # CellCapture = claim + winners + apply + settle + the rebuild of the next list (the reference's pass 1
     # and pass 2)
     cap_ms = sum(v for k, v in kms.items() if k != "k_nucleate")
     bytes_alg = 137 * prof["frontier"] + 125 * prof["captures"] + 1 * prof["solidified"]
@@ -311,7 +311,7 @@ def main() -> int:
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "CellCapture = k_claim + k_apply + k_settle + k_rebuild (one step)",
+            "kernel": "CellCapture = k_claim + k_win + k_apply + k_settle + k_rebuild (one step)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",

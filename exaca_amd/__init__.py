@@ -381,10 +381,10 @@ class Solver:
 
     def profile_kernels(self, nsteps: int, layer: int = 0) -> dict:
         """Runs nsteps more steps with an event pair around every kernel; summed times in ms."""
-        ms = (ctypes.c_double * 5)()
+        ms = (ctypes.c_double * 6)()
         cnt = (ctypes.c_int64 * 5)()
         self._ok(self._lib.exaca_profile_kernels(self._h, layer, nsteps, ms, cnt))
-        names = ["k_nucleate", "k_claim", "k_apply", "k_settle", "k_rebuild"]
+        names = ["k_nucleate", "k_claim", "k_win", "k_apply", "k_settle", "k_rebuild"]
         return {"ms": dict(zip(names, list(ms))), "frontier": cnt[0], "captures": cnt[1], "solidified": cnt[2],
                 "steps": cnt[4]}
 

@@ -20,11 +20,11 @@
 //   lid  int8    LayerID
 //   ct   uint8   cell type (values of src/CAtypes.hpp:11) | kSeedBit for cell-centred octahedra
 //   flags        kCounted, kFresh
-// The octahedron centre (double, as computed at capture; DOCenter is its float rounding) is a
-// second 32-byte record per cell, read only when a cell evaluates critical lengths or captures.
-// The reference's 26 critical diagonal lengths per cell (104 B) are not stored: they are
-// recomputed bit-exactly from the double centre (captured cells) or read from a per-orientation
-// table (cell-centred octahedra: nuclei and the initial substrate).
+// The octahedron is a second record per cell, one aligned 128-byte line: the centre in double (as
+// computed at capture; DOCenter is its float rounding) and, for a captured cell, the reference's
+// 26 critical diagonal lengths, computed once by the capture. A cell-centred octahedron (nuclei,
+// the initial substrate, layer seeds) reads its lengths from a per-orientation table instead. A
+// frontier cell's evaluation reads the one line of lengths; a capture reads the claimant's centre.
 #pragma once
 #include <cstddef>
 #include <cstdint>
@@ -155,10 +155,14 @@ struct alignas(32) Cell {
     uint16_t flags;  // kCounted: past its liquidus time and counted in StepCtl::nonwaiting; kFresh
 };
 static_assert(sizeof(Cell) == 32, "one sector per cell");
-struct alignas(32) Centre {
-    double x, y, z, pad;
+struct alignas(128) Octa {
+    float cdl[26];  // CritDiagonalLength in the neighbour order kNbX/kNbY/kNbZ (captured cells)
+    double x, y, z; // centre
 };
-static_assert(sizeof(Centre) == 32, "one sector per cell");
+static_assert(sizeof(Octa) == 128, "one line per cell");
+// Row stride (floats) of the per-orientation table of cell-centred critical lengths: rows are
+// aligned like the lengths of an Octa record.
+constexpr int kCdlStride = 32;
 
 // Per-cell fields a caller can read back (values of EXACA_FIELD_* in the C interface).
 enum FieldKind : int {
@@ -197,7 +201,7 @@ struct DomainView {
     int xoff, yoff;         // MyXOffset, MyYOffset (-1 for one rank)
     long long ncell;        // X*Y*nz
     Cell *cells;
-    Centre *centre;
+    Octa *octa;             // octahedron records (centre, critical lengths)
     uint32_t *list[2];      // frontier of a step of parity p (list[1-p] also holds the step's
                             // solidification candidates until the next frontier is written)
     uint32_t *claim_cell;   // cells with claims in the current step, stored in their own frontier
@@ -212,7 +216,8 @@ struct DomainView {
     const int32_t *nloc;
     int32_t *ntime;         // NucleationTimes; a site examined past its time is set to INT32_MAX
     const float *guv;       // 10000 x 9 unit-vector components (GrainUnitVector)
-    const float *seed_cdl;  // 10000 x 26 critical diagonal lengths of cell-centred octahedra
+    const float *seed_cdl;  // 10000 rows of kCdlStride: the 26 critical diagonal lengths of a
+                            // cell-centred octahedron of each orientation
     const double *grow;     // growth increment after k executed steps below the liquidus
     int32_t ngrow;          // table length; the last entry holds for every larger k
     int32_t ksat;           // from this k on every increment is the last entry (the 0.045 cap or the

@@ -6,8 +6,8 @@
 //               every liquid neighbour it reaches (atomicMin of its index into the neighbour's
 //               claim word), and the step at which it is due next; a cell past its liquidus time
 //               with no liquid neighbour turns solid
-//   k_apply     the winner of each claimed cell builds the new decentred octahedron and takes the
-//               cell out of its neighbours' liquid masks
+//   k_apply     the winner of each claimed cell builds the new decentred octahedron, stores its
+//               26 critical lengths, and takes the cell out of its neighbours' liquid masks
 //   k_settle    a cell whose liquid neighbours were all taken by lower-indexed cells turns solid
 //   k_rebuild   the list of the cells due at the next step, in index order, from its calendar slot
 // A cell is due when something can happen to it: it is new, its liquidus time has passed, or its
@@ -161,6 +161,14 @@ __device__ __forceinline__ float critical_length(const Faces &f, double xp, doub
     const double d2 = ex * f.fx[2] + ey * f.fy[2] + ez * f.fz[2];
     const double d3 = ex * f.fx[3] + ey * f.fy[3] + ez * f.fz[3];
     return static_cast<float>(dmax(dmax(fabs(d0), fabs(d1)), dmax(fabs(d2), fabs(d3))));
+}
+
+// The centre of cell q's octahedron record (the last 24 bytes of its line).
+__device__ __forceinline__ void set_centre(const DomainView &d, uint32_t q, double x, double y, double z) {
+    Octa &o = d.octa[q];
+    o.x = x;
+    o.y = y;
+    o.z = z;
 }
 
 struct Octahedron {
@@ -472,8 +480,10 @@ __device__ __forceinline__ void solidify(const DomainView &d, const Growth &g, u
 // this step's capture and turns solid here.
 // Most steps have no site due: next_nuc[p], the least time of the sites not yet decided, lets every
 // workgroup return at once (its counter rows zero). A step that examines sites sets the next step's
-// value from what is left (next_nuc[1-p], reset to INT32_MAX by the capture pass of the step before),
-// a step that does not passes its own on.
+// value from what is left (next_nuc[1-p], reset to INT32_MAX by the capture pass of the step before);
+// for a step that does not, the capture pass passes the value on. The step's control words are
+// advanced by the capture pass too, so a step may leave this kernel out when no site can be due
+// (the host knows the sites' times).
 constexpr int kPartNucRows[4] = {kPartNucNn, kPartNucAct, kPartNucSol, kPartNucEntered};
 __device__ __forceinline__ void nucleate_body(const DomainView &d, int p, int layer, uint32_t b, uint32_t nb) {
     __shared__ unsigned long long sh[4 * (kBlock / 64)];
@@ -482,13 +492,6 @@ __device__ __forceinline__ void nucleate_body(const DomainView &d, int p, int la
     if (ctl->done) return;
     const int cycle = ctl->cycle[p];
     const int gate = ctl->next_nuc[p];
-    if (b == 0 && threadIdx.x == 0) {
-        ctl->cycle[1 - p] = cycle + 1;
-        ctl->serial[1 - p] = ctl->serial[p] + 1;
-        ctl->ncand = 0;
-        ctl->rebuild_seq = ctl->rebuild_seq + 1;
-        if (cycle < gate) ctl->next_nuc[1 - p] = gate;
-    }
     if (cycle < gate) {
         if (threadIdx.x < 4) d.part[static_cast<size_t>(kPartNucRows[threadIdx.x]) * d.nparts + b] = 0;
         return;
@@ -523,7 +526,7 @@ __device__ __forceinline__ void nucleate_body(const DomainView &d, int p, int la
         cq.lm = kLmUnknown;
         cq.flags = 0;
         cq.ct = kActive | kSeedBit;
-        d.centre[q] = Centre{(c.x + d.xoff) + 0.5, (c.y + d.yoff) + 0.5, c.z + 0.5, 0.0};
+        set_centre(d, q, (c.x + d.xoff) + 0.5, (c.y + d.yoff) + 0.5, c.z + 0.5);
         const uint32_t valid = valid_neighbours(c, gr.X, gr.Y, gr.nzA);
         // all updates in flight together, then the results
         uint32_t old[26];
@@ -566,47 +569,43 @@ __device__ __forceinline__ void nucleate_body(const DomainView &d, int p, int la
 
 // Critical lengths of the liquid neighbours of frontier cell j (half-diagonal dlj): claims on
 // those reached. thr becomes the least length of the others and is returned with the claimed
-// mask.
+// mask. The 26 lengths are one aligned 128-byte row: the cell's own (stored by its capture) or,
+// for a cell-centred octahedron, its orientation's row of the seed table. They are compared all
+// at once and the claims issued after, so a lane's work does not depend on its liquid count.
 __device__ uint32_t evaluate_claims(const DomainView &d, const Grid &gr, uint32_t j, float dlj, uint32_t liquid,
                                     int32_t gid, uint8_t ct, float &thr) {
-    int o = orientation_of(gid);
-    if (o < 0) {
-        atomicOr(&d.ctl->error, 2);
-        o = 0;
+    const float *row = d.octa[j].cdl;
+    if (ct & kSeedBit) {
+        int o = orientation_of(gid);
+        if (o < 0) {
+            atomicOr(&d.ctl->error, 2);
+            o = 0;
+        }
+        row = d.seed_cdl + static_cast<size_t>(kCdlStride) * o;
     }
+    float cdl[26];
+    const float4 *r4 = reinterpret_cast<const float4 *>(row);
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const float4 v = r4[i];
+        cdl[4 * i] = v.x;
+        cdl[4 * i + 1] = v.y;
+        cdl[4 * i + 2] = v.z;
+        cdl[4 * i + 3] = v.w;
+    }
+    const float2 v2 = reinterpret_cast<const float2 *>(row)[12];
+    cdl[24] = v2.x;
+    cdl[25] = v2.y;
     uint32_t claimed = 0;
     float next = __builtin_inff();
-    if (ct & kSeedBit) {
-        const float *tab = d.seed_cdl + 26 * o;
-        for (uint32_t m = liquid; m; m &= m - 1) {
-            const int l = __builtin_ctz(m);
-            const float cdl = tab[l];
-            if (dlj >= cdl) {
-                atomicMin(claim_word(d, j + gr.offset(l)), j);
-                claimed |= 1u << l;
-            }
-            else {
-                next = fminf(next, cdl);
-            }
-        }
+#pragma unroll
+    for (int l = 0; l < 26; l++) {
+        const bool liq = (liquid >> l) & 1u;
+        const bool reach = dlj >= cdl[l];
+        claimed |= static_cast<uint32_t>(liq && reach) << l;
+        next = (liq && !reach) ? fminf(next, cdl[l]) : next;
     }
-    else {
-        const Coord c = decode(j, gr);
-        const Centre e = d.centre[j];
-        const Faces f = faces_of(d.guv + 9 * o);
-        const double xp = (c.x + d.xoff) + 0.5, yp = (c.y + d.yoff) + 0.5, zp = c.z + 0.5;
-        for (uint32_t m = liquid; m; m &= m - 1) {
-            const int l = __builtin_ctz(m);
-            const float cdl = critical_length(f, xp, yp, zp, e.x, e.y, e.z, l);
-            if (dlj >= cdl) {
-                atomicMin(claim_word(d, j + gr.offset(l)), j);
-                claimed |= 1u << l;
-            }
-            else {
-                next = fminf(next, cdl);
-            }
-        }
-    }
+    for (uint32_t m = claimed; m; m &= m - 1) atomicMin(claim_word(d, j + gr.offset(__builtin_ctz(m))), j);
     d.cells[j].thr = next;
     thr = next;
     return claimed;
@@ -634,11 +633,20 @@ __device__ __forceinline__ void claim_body(const DomainView &d, int p, int layer
     StepCtl *ctl = d.ctl;
     if (ctl->done) return;
     if (threadIdx.x == 0) nclaim = 0;
-    // this step's nucleation read its gate: reset it for the nucleation two steps on
-    if (b == 0 && threadIdx.x == 0) ctl->next_nuc[p] = 0x7FFFFFFF;
     const Growth g = staged_growth(d, gtab);
     const int cycle = ctl->cycle[p];
     const uint32_t serial = ctl->serial[p];
+    if (b == 0 && threadIdx.x == 0) {
+        // the next step's control words (read only by the kernels after this one); this step's
+        // nucleation gate, passed on when no site was due, then reset for the step two on
+        ctl->cycle[1 - p] = cycle + 1;
+        ctl->serial[1 - p] = serial + 1;
+        ctl->ncand = 0;
+        ctl->rebuild_seq = ctl->rebuild_seq + 1;
+        const int gate = ctl->next_nuc[p];
+        if (cycle < gate) ctl->next_nuc[1 - p] = gate;
+        ctl->next_nuc[p] = 0x7FFFFFFF;
+    }
     uint32_t begin, end;
     frontier_slice(ctl->count[p], d.list_cap, b, nb, begin, end);
     const uint32_t *list = d.list[p];
@@ -733,19 +741,54 @@ __device__ __forceinline__ void claim_body(const DomainView &d, int p, int layer
     put_parts4(d, {kPartClaimSol, kPartClaimEntered, kPartClaimEval, kPartClaimMask}, v4, b, sh);
 }
 
-// Pass B over the cells with claims. Workgroup a takes the claims of the kApplyGroup slices
-// a*kApplyGroup .. of pass A's nbc workgroups (a slice holds a few dozen claims: grouping fills the
-// waves and keeps the whole grid resident). A claim whose word holds the claimant's own index is won
-// (the lowest index that reached the cell). The won captures of a round of claimants are numbered
-// by a block scan and then built one per lane: the winner's octahedron gives the captured cell's
+// Pass B1 over the cells with claims, one workgroup per pass-A slice: a claim whose word holds the
+// claimant's own index is won (the lowest index that reached the cell). The slice's claimants that
+// won anything are compacted in place, with the claims they won, and slice_claims[b] becomes their
+// count. (Winners of a round go to positions below the round's end, whose entries every lane has
+// read before the barrier; later rounds' entries lie above.) Each claimant needs only its claim
+// words here: one round of every slice is resident at once, a short chain of dependent loads.
+__device__ __forceinline__ void win_body(const DomainView &d, int p, uint32_t b, uint32_t nb) {
+    __shared__ uint32_t nwin;
+    StepCtl *ctl = d.ctl;
+    if (ctl->done) return;
+    uint32_t beg, e;
+    frontier_slice(ctl->count[p], d.list_cap, b, nb, beg, e);
+    const uint32_t n = d.slice_claims[b];
+    const Grid gr = grid_of(d);
+    if (threadIdx.x == 0) nwin = 0;
+    for (uint32_t k0 = 0; k0 < n; k0 += kBlock) {
+        const uint32_t k = k0 + threadIdx.x;
+        uint32_t j = 0, won = 0;
+        if (k < n) {
+            j = d.claim_cell[beg + k];
+            const uint32_t cb = d.claim_bits[beg + k];
+            uint32_t cw[26];
+#pragma unroll
+            for (int l = 0; l < 26; l++) cw[l] = ((cb >> l) & 1u) ? *claim_word(d, j + gr.off[l]) : kClaimNone;
+#pragma unroll
+            for (int l = 0; l < 26; l++) won |= static_cast<uint32_t>(cw[l] == j) << l;
+        }
+        __syncthreads();
+        const uint32_t slot = wave_append(won != 0, &nwin);
+        if (won) {
+            d.claim_cell[beg + slot] = j;
+            d.claim_bits[beg + slot] = won;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) d.slice_claims[b] = nwin;
+}
+
+// Pass B2 over the winners: workgroup a takes the winners of the kApplyGroup slices a*kApplyGroup ..
+// (a few dozen each). The captures of a round of winners are numbered by a block scan and built one
+// per lane (a winner mostly has one; one lane per capture keeps every lane's chain of dependent
+// accesses to one capture): the winner's octahedron gives the captured cell's
 // (src/CAupdate.cpp:353-563, 717), the new cell is due at the next step, and the captured cell
 // leaves the liquid masks of its neighbours. Only masks that still hold it are updated: the masks
 // are read first, and a cell's bit is cleared by its capture alone, so the read decides exactly.
 // A neighbour whose mask this empties is a solidification candidate of this step, staged per
 // workgroup in LDS and written to the workgroup's own region of the candidate buffer (no shared
 // counter); a wave that finds the stage full appends to the overflow list in the other list buffer.
-
-
 __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t a, uint32_t nbc) {
     __shared__ unsigned long long sh[kBlock / 64];
     __shared__ uint32_t wave_tot[kBlock / 64];
@@ -783,7 +826,8 @@ __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t 
     unsigned long long ncap = 0;
     uint32_t qlo = 0xFFFFFFFFu, qhi = 0;
     for (uint32_t k0 = 0; k0 < n; k0 += kBlock) {
-        // Won claims of this round's claimants, and their block-wide exclusive prefix.
+        // This round's winners, their grain and centre (float, as the capture uses it) in LDS, and
+        // the block-wide exclusive prefix of their capture counts.
         const uint32_t k = k0 + threadIdx.x;
         uint32_t j = 0, won = 0;
         if (k < n) {
@@ -792,20 +836,12 @@ __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t 
             for (uint32_t g = 1; g < kApplyGroup; g++)
                 if (k >= upto[g - 1]) i = beg[g] + (k - upto[g - 1]);
             j = d.claim_cell[i];
-            // the claimant's grain and centre (float, as the capture uses it) travel through LDS
-            // to the lanes that build its captures
-            const Centre e = d.centre[j];
+            won = d.claim_bits[i];
+            const Octa &e = d.octa[j];
             won_gid[threadIdx.x] = d.cells[j].gid;
             won_cx[threadIdx.x] = static_cast<float>(e.x);
             won_cy[threadIdx.x] = static_cast<float>(e.y);
             won_cz[threadIdx.x] = static_cast<float>(e.z);
-            // the claim words of every claimed neighbour, loaded together
-            const uint32_t cb = d.claim_bits[i];
-            uint32_t cw[26];
-#pragma unroll
-            for (int l = 0; l < 26; l++) cw[l] = ((cb >> l) & 1u) ? *claim_word(d, j + gr.off[l]) : kClaimNone;
-#pragma unroll
-            for (int l = 0; l < 26; l++) won |= static_cast<uint32_t>(cw[l] == j) << l;
         }
         const uint32_t cnt = __builtin_popcount(won);
         uint32_t wtot;
@@ -822,10 +858,10 @@ __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t 
         pre[threadIdx.x] = base + incl - cnt;
         if (threadIdx.x == 0) pre[kBlock] = total;
         __syncthreads();
-        // One capture per lane: item t belongs to the claimant s with pre[s] <= t < pre[s + 1].
+        // One capture per lane: item t belongs to the winner s with pre[s] <= t < pre[s + 1].
         for (uint32_t t0 = 0; t0 < total; t0 += kBlock) {
             const uint32_t t = t0 + threadIdx.x;
-            uint32_t q = 0, emptied = 0, jc = 0;
+            uint32_t q = 0, emptied = 0;
             if (t < total) {
                 uint32_t lo = 0, hi = kBlock;  // pre[lo] <= t < pre[hi]
                 while (hi - lo > 1) {
@@ -835,17 +871,18 @@ __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t 
                     else
                         hi = mid;
                 }
-                jc = won_cell[lo];
+                const uint32_t jc = won_cell[lo];
                 uint32_t bits = won_bits[lo];
                 for (uint32_t r = t - pre[lo]; r; r--) bits &= bits - 1;
                 const int l = __builtin_ctz(bits);
                 q = jc + gr.offset(l);
+                const int h = won_gid[lo];
+                const float wcx = won_cx[lo], wcy = won_cy[lo], wcz = won_cz[lo];
                 const Coord c = decode(jc, gr);
                 const Coord cqc{c.x + nb_dx(l), c.y + nb_dy(l), c.z + nb_dz(l)};
                 const uint32_t valid = valid_neighbours(cqc, gr.X, gr.Y, gr.nzA);
                 // the grain's orientation, loaded with the neighbours' words below rather than
                 // after the mask atomics
-                const int h = won_gid[lo];
                 int o = orientation_of(h);
                 if (o < 0) {
                     atomicOr(&ctl->error, 2);
@@ -891,14 +928,13 @@ __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t 
                 const double xp = (c.x + d.xoff + nb_dx(l)) + 0.5;
                 const double yp = (c.y + d.yoff + nb_dy(l)) + 0.5;
                 const double zp = (c.z + nb_dz(l)) + 0.5;
-                const Octahedron oc = new_octahedron(gv, won_cx[lo], won_cy[lo], won_cz[lo], xp, yp, zp);
+                const Octahedron oc = new_octahedron(gv, wcx, wcy, wcz, xp, yp, zp);
                 Cell &cq = d.cells[q];
                 cq.gid = h;
                 cq.dl = static_cast<float>(oc.dl);
                 cq.thr = 0.0f;
                 cq.flags = kFresh;
                 cq.ct = kActive;
-                d.centre[q] = Centre{oc.cx, oc.cy, oc.cz, 0.0};
                 set_alarm(d, serial + 1, q);
                 ncap++;
                 qlo = q < qlo ? q : qlo;
@@ -909,6 +945,24 @@ __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t 
                     emptied |= static_cast<uint32_t>(((need >> kk) & 1u) && (mw[kk] & (kMaskUnknown | kMaskNow)) == bit) << kk;
                 }
                 cq.lm = liquid;
+                // The new octahedron's record: its 26 critical lengths, computed once here (its
+                // evaluations read them back), and its centre; whole 16-byte pieces of one line.
+                {
+                    const Faces f = faces_of(gv);
+                    float4 *o4 = reinterpret_cast<float4 *>(&d.octa[q]);
+#pragma unroll
+                    for (int i = 0; i < 6; i++)
+                        o4[i] = make_float4(critical_length(f, xp, yp, zp, oc.cx, oc.cy, oc.cz, 4 * i),
+                                            critical_length(f, xp, yp, zp, oc.cx, oc.cy, oc.cz, 4 * i + 1),
+                                            critical_length(f, xp, yp, zp, oc.cx, oc.cy, oc.cz, 4 * i + 2),
+                                            critical_length(f, xp, yp, zp, oc.cx, oc.cy, oc.cz, 4 * i + 3));
+                    const uint2 xb = __builtin_bit_cast(uint2, oc.cx), yb = __builtin_bit_cast(uint2, oc.cy),
+                                zb = __builtin_bit_cast(uint2, oc.cz);
+                    uint4 *u4 = reinterpret_cast<uint4 *>(&d.octa[q]);
+                    u4[6] = make_uint4(__float_as_uint(critical_length(f, xp, yp, zp, oc.cx, oc.cy, oc.cz, 24)),
+                                       __float_as_uint(critical_length(f, xp, yp, zp, oc.cx, oc.cy, oc.cz, 25)), xb.x, xb.y);
+                    u4[7] = make_uint4(yb.x, yb.y, zb.x, zb.y);
+                }
             }
             // Candidates: staged in LDS (one LDS atomic per wave); a wave that finds the stage
             // full appends to the global list directly. Waves reserve disjoint stage ranges in
@@ -943,7 +997,7 @@ __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t 
         __syncthreads();
     }
     // The staged candidates go to this workgroup's region (the barrier orders thread 0's
-    // initialisation before these reads when the workgroup had no claims).
+    // initialisation before these reads when the workgroup had no winners).
     __syncthreads();
     const uint32_t staged = ncstage < stage_end ? ncstage : stage_end;
     for (uint32_t i = threadIdx.x; i < staged; i += kBlock) d.cand[a * kCandStage + i] = cstage[i];
@@ -1051,7 +1105,7 @@ __device__ __forceinline__ uint32_t *next_slot(const DomainView &d, int p) {
     return d.alarms + ((d.ctl->serial[p] + 1) % kAlarmSlots) * alarm_stride(d.ncell);
 }
 // Flags of one rebuild carry its tag (the rebuild sequence number, advanced by the kernels that
-// precede a rebuild: nucleation and rewake; tags cycle through 2^30 - 1 nonzero values), so the
+// precede a rebuild: the capture pass, rewake and layer setup; tags cycle through 2^30 - 1 nonzero values), so the
 // flags of earlier rebuilds need no clearing.
 __device__ __forceinline__ unsigned long long flag_tag(uint32_t seq) {
     return static_cast<unsigned long long>(seq % ((1u << 30) - 1) + 1) << 34;
@@ -1372,7 +1426,7 @@ __global__ __launch_bounds__(kBlock) void k_layer_setup(DomainView d, int p_next
         c.win = 0;
         c.flags = 0;
         c.ct = kActive | kSeedBit;
-        d.centre[q] = Centre{(p.x + d.xoff) + 0.5, (p.y + d.yoff) + 0.5, p.z + 0.5, 0.0};
+        set_centre(d, q, (p.x + d.xoff) + 0.5, (p.y + d.yoff) + 0.5, p.z + 0.5);
         set_alarm(d, serial, static_cast<uint32_t>(q));
         n++;
         qlo = static_cast<uint32_t>(q) < qlo ? static_cast<uint32_t>(q) : qlo;
@@ -1399,6 +1453,9 @@ __global__ __launch_bounds__(kBlock) void k_nucleate(DomainView d, int p, int la
 __global__ __launch_bounds__(kBlock) void k_claim(DomainView d, int p, int layer) {
     claim_body(d, p, layer, blockIdx.x, gridDim.x);
 }
+__global__ __launch_bounds__(kBlock) void k_win(DomainView d, int p) {
+    win_body(d, p, blockIdx.x, gridDim.x);
+}
 __global__ __launch_bounds__(kBlock) void k_apply(DomainView d, int p, int nbc) {
     apply_body(d, p, blockIdx.x, static_cast<uint32_t>(nbc));
 }
@@ -1421,6 +1478,10 @@ __global__ __launch_bounds__(kBlock) void k_nucleate_n(const DomainView *v, int 
 __global__ __launch_bounds__(kBlock) void k_claim_n(const DomainView *v, int p, int layer) {
     const DomainView &d = v[blockIdx.y];
     if (d.ctl) claim_body(d, p, layer, blockIdx.x, gridDim.x);
+}
+__global__ __launch_bounds__(kBlock) void k_win_n(const DomainView *v, int p) {
+    const DomainView &d = v[blockIdx.y];
+    if (d.ctl) win_body(d, p, blockIdx.x, gridDim.x);
 }
 __global__ __launch_bounds__(kBlock) void k_apply_n(const DomainView *v, int p, int nbc) {
     const DomainView &d = v[blockIdx.y];
@@ -1503,7 +1564,7 @@ __global__ __launch_bounds__(kBlock) void k_seed_init(DomainView d, const uint32
         const uint32_t q = cells[i];
         const Coord c = decode(q, d.X, d.Y);
         d.cells[q].dl = 0.01f;
-        d.centre[q] = Centre{(c.x + d.xoff) + 0.5, (c.y + d.yoff) + 0.5, c.z + 0.5, 0.0};
+        set_centre(d, q, (c.x + d.xoff) + 0.5, (c.y + d.yoff) + 0.5, c.z + 0.5);
     }
 }
 
@@ -1583,7 +1644,7 @@ __global__ __launch_bounds__(kBlock) void k_init_directional(DomainView d, const
         uint4 *r = reinterpret_cast<uint4 *>(d.cells + q);
         r[0] = make_uint4(wall ? kMaskUnknown : 0xFFFFFFFFu, wall ? 0u : kClaimNone, 0u, 0u);
         r[1] = make_uint4(static_cast<uint32_t>(plane_cts[c.z]), 0u, 0u, static_cast<uint32_t>(ct) << 8);
-        d.centre[q] = Centre{0.0, 0.0, 0.0, 0.0};
+        set_centre(d, q, 0.0, 0.0, 0.0);
         if (melted) melted[q] = wall ? 0 : 1;
     }
 }
@@ -1597,7 +1658,7 @@ __global__ __launch_bounds__(kBlock) void k_init_substrate(DomainView d, const u
         r.dl = 0.01f;
         r.gid = static_cast<int32_t>(i + 1);
         r.ct = kActive | kSeedBit;
-        d.centre[q] = Centre{(c.x + d.xoff) + 0.5, (c.y + d.yoff) + 0.5, c.z + 0.5, 0.0};
+        set_centre(d, q, (c.x + d.xoff) + 0.5, (c.y + d.yoff) + 0.5, c.z + 0.5);
     }
 }
 __global__ __launch_bounds__(kBlock) void k_init_sites(DomainView d, const int32_t *sites, uint32_t n) {
@@ -1617,7 +1678,7 @@ __global__ __launch_bounds__(kBlock) void k_pack(DomainView d, const uint8_t *ct
         c[0] = make_uint4(liquid ? 0xFFFFFFFFu : kMaskUnknown, liquid ? kClaimNone : 0u, 0u, 0u);
         c[1] = make_uint4(static_cast<uint32_t>(cts[q]), static_cast<uint32_t>(gid[q]), 0u,
                           static_cast<uint32_t>(static_cast<uint8_t>(lid[q])) | (static_cast<uint32_t>(ct[q]) << 8));
-        d.centre[q] = Centre{0.0, 0.0, 0.0, 0.0};
+        set_centre(d, q, 0.0, 0.0, 0.0);
     }
 }
 
@@ -1640,7 +1701,7 @@ __global__ __launch_bounds__(kBlock) void k_extract(DomainView d, int field, voi
         case kFieldCritTimeStep: static_cast<int32_t *>(out)[q] = c.cts; break;
         case kFieldLayerId: static_cast<int32_t *>(out)[q] = c.lid; break;
         case kFieldCentre: {  // float x, y, z
-            const Centre &e = d.centre[q];
+            const Octa &e = d.octa[q];
             float *o = static_cast<float *>(out) + 3 * q;
             o[0] = static_cast<float>(e.x);
             o[1] = static_cast<float>(e.y);
@@ -1648,7 +1709,7 @@ __global__ __launch_bounds__(kBlock) void k_extract(DomainView d, int field, voi
             break;
         }
         case kFieldCentreF64: {  // double x, y, z, as computed at capture
-            const Centre &e = d.centre[q];
+            const Octa &e = d.octa[q];
             double *o = static_cast<double *>(out) + 3 * q;
             o[0] = e.x;
             o[1] = e.y;
@@ -1734,11 +1795,12 @@ uint32_t rebuild_units(long long ncell) {
     const unsigned long long chunks = (words + kChunkWords - 1) / kChunkWords;
     return static_cast<uint32_t>((chunks + kUnitChunks - 1) / kUnitChunks);
 }
-void launch_apply(const DomainView &d, int parity, int layer, int blocks, hipStream_t s) {
+void launch_apply(const DomainView &d, int parity, int layer, int blocks, bool nucleated, hipStream_t s) {
     const int cb = rebuild_blocks(d, blocks);
+    k_win<<<blocks, kBlock, 0, s>>>(d, parity);
     k_apply<<<apply_blocks(blocks), kBlock, 0, s>>>(d, parity, blocks);
     k_settle<<<settle_blocks(blocks), kBlock, 0, s>>>(d, parity, layer);
-    k_rebuild<<<cb, kBlock, 0, s>>>(d, parity, 1, blocks, grid_for(d.nnuclei, 1024));
+    k_rebuild<<<cb, kBlock, 0, s>>>(d, parity, 1, blocks, nucleated ? grid_for(d.nnuclei, 1024) : 0);
 }
 void launch_layer_bounds(const DomainView &d, int next_layer, int *out, hipStream_t s) {
     k_layer_bounds<<<grid_for(d.ncell, 4096), kBlock, 0, s>>>(d, next_layer, out);
@@ -1765,6 +1827,9 @@ void launch_step_timed(const DomainView &d, int parity, int layer, int blocks, h
     k_claim<<<blocks, kBlock, 0, s>>>(d, parity, layer);
     rec();
     rec();
+    k_win<<<blocks, kBlock, 0, s>>>(d, parity);
+    rec();
+    rec();
     k_apply<<<apply_blocks(blocks), kBlock, 0, s>>>(d, parity, blocks);
     rec();
     rec();
@@ -1778,6 +1843,7 @@ void launch_step_n(const DomainView *views, int slots, int parity, int layer, in
                    hipStream_t s) {
     k_nucleate_n<<<dim3(nuc_blocks, slots), kBlock, 0, s>>>(views, parity, layer);
     k_claim_n<<<dim3(blocks, slots), kBlock, 0, s>>>(views, parity, layer);
+    k_win_n<<<dim3(blocks, slots), kBlock, 0, s>>>(views, parity);
     k_apply_n<<<dim3(apply_blocks(blocks), slots), kBlock, 0, s>>>(views, parity, blocks);
     k_settle_n<<<dim3(settle_blocks(blocks), slots), kBlock, 0, s>>>(views, parity, layer);
     // one round of workgroups covers every unit of an ensemble domain (at most kEnsembleUnits):

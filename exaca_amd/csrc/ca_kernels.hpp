@@ -8,13 +8,18 @@ namespace exaca {
 
 constexpr int kBlock = 256;  // four wave64s per workgroup
 
-// Nucleation (src/CAupdate.cpp:67-200). Also prepares the control slot of the next step.
+// Nucleation (src/CAupdate.cpp:67-200). A step may leave it out when no site's time falls in the
+// steps since nucleation last ran (it would examine nothing); its capture pass is then launched
+// with nucleated = false.
 void launch_nucleation(const DomainView &d, int parity, int layer, hipStream_t s);
 // CellCapture pass A: liquidus-time gate, growth, neighbour scan, claims (src/CAupdate.cpp:217-352).
+// Also prepares the control slot of the next step.
 void launch_claim(const DomainView &d, int parity, int layer, int blocks, hipStream_t s);
-// CellCapture pass B: winners build new octahedra, cells left without a liquid neighbour by
-// lower-indexed captures solidify, and the next frontier is written (src/CAupdate.cpp:353-725).
-void launch_apply(const DomainView &d, int parity, int layer, int blocks, hipStream_t s);
+// CellCapture pass B: claims decided, winners build new octahedra, cells left without a liquid
+// neighbour by lower-indexed captures solidify, and the next frontier is written
+// (src/CAupdate.cpp:353-725).
+// nucleated: whether this step's nucleation kernel ran (its counter rows are booked).
+void launch_apply(const DomainView &d, int parity, int layer, int blocks, bool nucleated, hipStream_t s);
 // Every active cell due at the next step, and that step's list rebuilt: after a cycle jump, whose
 // skipped steps the calendar did not count. parity_next is the parity of the next step.
 void launch_rewake(const DomainView &d, int parity_next, int blocks, hipStream_t s);
@@ -30,9 +35,9 @@ void launch_step_n(const DomainView *views, int slots, int parity, int layer, in
 void launch_check(const DomainView &d, int cycle, int layer, CheckResult *out, hipStream_t s);
 // Octahedra of the initially active cells (src/CAinitialize.cpp:1936-2043).
 void launch_seed_init(const DomainView &d, const uint32_t *cells, uint32_t n, hipStream_t s);
-// One step with an event pair around each kernel: nucleation, claim, apply, settle, and the
-// frontier rebuild. ev holds 2 * kTimedKernels events.
-constexpr int kTimedKernels = 5;
+// One step with an event pair around each kernel: nucleation, claim, winners, apply, settle, and
+// the frontier rebuild. ev holds 2 * kTimedKernels events.
+constexpr int kTimedKernels = 6;
 void launch_step_timed(const DomainView &d, int parity, int layer, int blocks, hipStream_t s, hipEvent_t *ev);
 // Initial cell records from compact device arrays (CellType with the seed bit, LayerID, GrainID,
 // CritTimeStep), and one field back out of the records (FieldKind layout).

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <chrono>
 #include <cstdlib>
+#include <limits>
 #include <memory>
 #include <mutex>
 #include <cstdio>
@@ -146,8 +147,16 @@ struct exaca_state {
     int pending = -1;      // cycle whose nucleation ran but whose capture has not
     int blocks = 2048;     // workgroups of the frontier kernels
     int njumps = 0;
-    hipGraphExec_t graph = nullptr;
+    hipGraphExec_t graph = nullptr;       // kGraphSteps steps
+    hipGraphExec_t graph_quiet = nullptr; // the same without the nucleation kernels
     int graph_layer = -1;
+    // Nucleation times in ascending order, and the last step whose nucleation kernel ran: a step
+    // may leave the kernel out when no time falls after that step and up to it (every site of a
+    // time at or before a step where the kernel ran is decided there). Single-layer decks only: a
+    // site of a later layer waits past its time.
+    std::vector<int32_t> nuc_times;
+    int nuc_through = 0;
+    bool nuc_skip = false;
     double init_seconds = 0;
     int device = 0;
     int layer = 0;              // the layer being solidified
@@ -165,6 +174,7 @@ struct exaca_state {
     }
     ~exaca_state() {
         if (graph) (void)hipGraphExecDestroy(graph);
+        if (graph_quiet) (void)hipGraphExecDestroy(graph_quiet);
         for (void *p : buffers) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -200,7 +210,7 @@ void build_device_state(exaca_state *h) {
     d.ncell = hd.ncell;
     const size_t n = static_cast<size_t>(hd.ncell);
     d.cells = h->alloc<Cell>(n);
-    d.centre = h->alloc<Centre>(n);
+    d.octa = h->alloc<Octa>(n);
     d.list_cap = static_cast<uint32_t>(n);
     d.list[0] = h->alloc<uint32_t>(n);
     d.list[1] = h->alloc<uint32_t>(n);
@@ -222,8 +232,8 @@ void build_device_state(exaca_state *h) {
     d.ntime = ntime;
     float *guv = h->alloc<float>(hd.guv.size());
     d.guv = guv;
-    std::vector<float> seed(26 * kNumOrientations);
-    for (int o = 0; o < kNumOrientations; o++) seed_critical_lengths(&hd.guv[9 * o], &seed[26 * o]);
+    std::vector<float> seed(static_cast<size_t>(kCdlStride) * kNumOrientations, 0.0f);
+    for (int o = 0; o < kNumOrientations; o++) seed_critical_lengths(&hd.guv[9 * o], &seed[kCdlStride * o]);
     float *seed_d = h->alloc<float>(seed.size());
     d.seed_cdl = seed_d;
     std::vector<double> grow = growth_table(hd.A, hd.B, hd.C, hd.D, hd.dU, 1 << 25);
@@ -309,10 +319,11 @@ void next_layer(exaca_state *h, int layer_done, int verbose) {
     }
     h->dv.zlo = zlo;
     h->dv.nzA = zhigh + 1;
-    if (h->graph) {  // the graph holds the old window in its kernel arguments
-        HIP_OK(hipGraphExecDestroy(h->graph));
-        h->graph = nullptr;
-    }
+    for (hipGraphExec_t *ge : {&h->graph, &h->graph_quiet})  // they hold the old window in their arguments
+        if (*ge) {
+            HIP_OK(hipGraphExecDestroy(*ge));
+            *ge = nullptr;
+        }
     // the layer's record, the new layer's first cycle, and fresh counts of the active cells
     StepCtl *c = h->ctl;
     const int32_t end = h->cycle, j0 = h->njumps, layer = layer_done + 1, one = 1, zero = 0, open = 0;
@@ -340,28 +351,37 @@ void next_layer(exaca_state *h, int layer_done, int verbose) {
 void step_nucleation(exaca_state *h, int layer) {
     exaca::launch_nucleation(h->dv, h->parity, layer, h->stream);
 }
-void step_capture(exaca_state *h, int layer) {
+void step_capture(exaca_state *h, int layer, bool nucleated = true) {
     exaca::launch_claim(h->dv, h->parity, layer, h->blocks, h->stream);
-    exaca::launch_apply(h->dv, h->parity, layer, h->blocks, h->stream);
+    exaca::launch_apply(h->dv, h->parity, layer, h->blocks, nucleated, h->stream);
+}
+// Whether steps h->cycle + 1 .. last need the nucleation kernel.
+bool nucleation_due(const exaca_state *h, int last) {
+    if (!h->nuc_skip) return true;
+    const auto it = std::upper_bound(h->nuc_times.begin(), h->nuc_times.end(), h->nuc_through);
+    return it != h->nuc_times.end() && *it <= last;
 }
 
 void ensure_graph(exaca_state *h, int layer) {
     if (h->graph && h->graph_layer == layer) return;
-    if (h->graph) {
-        HIP_OK(hipGraphExecDestroy(h->graph));
-        h->graph = nullptr;
+    for (hipGraphExec_t *ge : {&h->graph, &h->graph_quiet})
+        if (*ge) {
+            HIP_OK(hipGraphExecDestroy(*ge));
+            *ge = nullptr;
+        }
+    for (bool nucleate : {true, false}) {
+        hipGraph_t g;
+        HIP_OK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < kGraphSteps; k++) {
+            const int p = k & 1;
+            if (nucleate) exaca::launch_nucleation(h->dv, p, layer, h->stream);
+            exaca::launch_claim(h->dv, p, layer, h->blocks, h->stream);
+            exaca::launch_apply(h->dv, p, layer, h->blocks, nucleate, h->stream);
+        }
+        HIP_OK(hipStreamEndCapture(h->stream, &g));
+        HIP_OK(hipGraphInstantiate(nucleate ? &h->graph : &h->graph_quiet, g, nullptr, nullptr, 0));
+        HIP_OK(hipGraphDestroy(g));
     }
-    hipGraph_t g;
-    HIP_OK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < kGraphSteps; k++) {
-        const int p = k & 1;
-        exaca::launch_nucleation(h->dv, p, layer, h->stream);
-        exaca::launch_claim(h->dv, p, layer, h->blocks, h->stream);
-        exaca::launch_apply(h->dv, p, layer, h->blocks, h->stream);
-    }
-    HIP_OK(hipStreamEndCapture(h->stream, &g));
-    HIP_OK(hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0));
-    HIP_OK(hipGraphDestroy(g));
     h->graph_layer = layer;
 }
 
@@ -369,13 +389,20 @@ void run_steps(exaca_state *h, int layer, int n, bool use_graph) {
     while (n > 0) {
         if (use_graph && h->parity == 0 && n >= kGraphSteps) {
             ensure_graph(h, layer);
-            HIP_OK(hipGraphLaunch(h->graph, h->stream));
-            h->cycle += kGraphSteps;
+            const int last = h->cycle + kGraphSteps;
+            const bool nucleate = nucleation_due(h, last);
+            HIP_OK(hipGraphLaunch(nucleate ? h->graph : h->graph_quiet, h->stream));
+            if (nucleate) h->nuc_through = last;
+            h->cycle = last;
             n -= kGraphSteps;
         }
         else {
-            step_nucleation(h, layer);
-            step_capture(h, layer);
+            const bool nucleate = nucleation_due(h, h->cycle + 1);
+            if (nucleate) {
+                step_nucleation(h, layer);
+                h->nuc_through = h->cycle + 1;
+            }
+            step_capture(h, layer, nucleate);
             h->parity ^= 1;
             h->cycle += 1;
             n -= 1;
@@ -481,6 +508,10 @@ exaca_state *create_state(const std::string &input, const std::string &dd, bool 
     h->deck.seed = seed;
     h->host = exaca::initialize_sparse(h->deck, verbose);
     build_device_state(h.get());
+    h->nuc_times = h->host.ntime;
+    std::sort(h->nuc_times.begin(), h->nuc_times.end());
+    h->nuc_through = std::numeric_limits<int>::min();
+    h->nuc_skip = h->deck.num_layers == 1 && std::getenv("EXACA_NUCLEATE_EVERY_STEP") == nullptr;
     h->init_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return h.release();
 }
@@ -504,6 +535,7 @@ void profile(exaca_state *h, int layer, int nsteps, double *ms, int64_t counts[5
             exaca::launch_step_timed(h->dv, h->parity, layer, h->blocks, h->stream, &ev[2 * kTimedKernels * k]);
             h->parity ^= 1;
             h->cycle += 1;
+            h->nuc_through = h->cycle;
             if (h->cycle % 1000 == 0) {
                 k++;
                 break;
@@ -695,6 +727,7 @@ int exaca_nucleation(exaca_state *h, int cycle, int layer) {
         step_nucleation(h, layer);
         HIP_OK(hipGetLastError());
         h->pending = cycle;
+        h->nuc_through = cycle;
     });
 }
 
@@ -864,7 +897,7 @@ int exaca_profile_steps(exaca_state *h, int layer, int nsteps, double ms[3], int
     });
 }
 
-int exaca_profile_kernels(exaca_state *h, int layer, int nsteps, double ms[5], int64_t counts[5]) {
+int exaca_profile_kernels(exaca_state *h, int layer, int nsteps, double ms[6], int64_t counts[5]) {
     return guarded([&] {
         if (!h || !ms || !counts || nsteps < 0) throw std::invalid_argument("bad argument");
         if (h->pending >= 0) throw StateError("profile_kernels called between nucleation and capture");

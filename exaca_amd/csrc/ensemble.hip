@@ -83,7 +83,7 @@ struct Arena {
 size_t domain_bytes(const exaca::HostDomain &h, size_t ngrow) {
     const size_t n = static_cast<size_t>(h.ncell);
     size_t b = 0;
-    b += up(sizeof(exaca::Cell) * n) + up(sizeof(exaca::Centre) * n);  // cell records
+    b += up(sizeof(exaca::Cell) * n) + up(sizeof(exaca::Octa) * n);  // cell records
     b += 4 * up(4 * n) + up(4 * kMaxSlices);                            // frontier, claim lists
     b += up(4ull * exaca::apply_blocks(kMaxSlices) * exaca::kCandStage) + up(4 * exaca::apply_blocks(kMaxSlices));
     b += 2 * up(4 * std::max<size_t>(h.nloc.size(), 1)) + up(8 * ngrow);
@@ -144,8 +144,8 @@ void ensure_tables(exaca_ensemble *e, const exaca::HostDomain &h, const std::str
         return;
     }
     e->orientation_file = orientation_file;
-    std::vector<float> seed(26 * kNumOrientations);
-    for (int o = 0; o < kNumOrientations; o++) seed_critical_lengths(&h.guv[9 * o], &seed[26 * o]);
+    std::vector<float> seed(static_cast<size_t>(kCdlStride) * kNumOrientations, 0.0f);
+    for (int o = 0; o < kNumOrientations; o++) seed_critical_lengths(&h.guv[9 * o], &seed[kCdlStride * o]);
     HIP_OK(hipMalloc(&e->guv, h.guv.size() * 4));
     HIP_OK(hipMalloc(&e->seed_cdl, seed.size() * 4));
     HIP_OK(hipMemcpy(e->guv, h.guv.data(), h.guv.size() * 4, hipMemcpyHostToDevice));
@@ -196,7 +196,7 @@ void admit(exaca_ensemble *e, const exaca_prepared *pr, int *slot_out) {
     d.yoff = hd.yoff;
     d.ncell = hd.ncell;
     d.cells = reinterpret_cast<Cell *>(carve(sizeof(Cell) * n));
-    d.centre = reinterpret_cast<Centre *>(carve(sizeof(Centre) * n));
+    d.octa = reinterpret_cast<Octa *>(carve(sizeof(Octa) * n));
     d.list[0] = reinterpret_cast<uint32_t *>(carve(4 * n));
     d.list[1] = reinterpret_cast<uint32_t *>(carve(4 * n));
     d.claim_cell = reinterpret_cast<uint32_t *>(carve(4 * n));

@@ -178,9 +178,9 @@ const char *exaca_last_error(void);
  * of steps executed (checks and cycle jumps are applied as in exaca_run_layer). */
 int exaca_profile_steps(exaca_state *h, int layer, int nsteps, double ms[3], int64_t counts[5]);
 
-/* As exaca_profile_steps with every kernel timed on its own: ms[0..4] = nucleation, claim, apply,
- * settle and the rebuild of the next step's list. */
-int exaca_profile_kernels(exaca_state *h, int layer, int nsteps, double ms[5], int64_t counts[5]);
+/* As exaca_profile_steps with every kernel timed on its own: ms[0..5] = nucleation, claim, the
+ * claims' winners, apply, settle and the rebuild of the next step's list. */
+int exaca_profile_kernels(exaca_state *h, int layer, int nsteps, double ms[6], int64_t counts[5]);
 
 /* HBM calibration (no reference counterpart): a plain device copy of `bytes` bytes repeated
  * `reps` times; *ms receives the average time of one copy. */

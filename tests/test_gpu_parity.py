@@ -123,6 +123,32 @@ def test_full_run_matches_oracle(case, tmp_path):
         _compare_state(sol, osim, case)
 
 
+def test_quiet_steps_leave_nucleation_out(tmp_path, monkeypatch):
+    """A single-layer run leaves the nucleation kernel out of every step (and 50-step graph) with
+    no nucleation time after the last step it ran; the capture pass advances the step's control
+    words and passes the nucleation gate on. The run must equal one that launches the kernel at
+    every step, counters and fields, and the oracle."""
+    deck = c_deck(tmp_path / "q.txt", 60, density=200, fraction=0.02)
+    runs = []
+    for every in (False, True):
+        if every:
+            monkeypatch.setenv("EXACA_NUCLEATE_EVERY_STEP", "1")
+        with ea.Solver(deck) as sol:
+            cycle, _ = sol.run()
+            q = sol.query()
+            runs.append((cycle, {k: q[k] for k in ("nn", "captures", "solidified", "frontier")},
+                         sol.field(ea.FIELD_CELLTYPE), sol.field(ea.FIELD_GRAINID), sol.field(ea.FIELD_DIAGONAL)))
+    (c0, q0, ct0, g0, d0), (c1, q1, ct1, g1, d1) = runs
+    assert q0["nn"] > 3, "the deck must nucleate"
+    assert c0 == c1 and q0 == q1
+    assert np.array_equal(ct0, ct1) and np.array_equal(g0, g1)
+    assert np.array_equal(d0.view(np.uint32), d1.view(np.uint32))
+    osim = orc.OracleSim(deck, DATA)
+    ocycle, _ = osim.run()
+    d = osim.dims()
+    assert c0 == ocycle and q0["nn"] == d["nn"] and q0["captures"] == d["captures"]
+
+
 @pytest.mark.parametrize("stage", [1, 3])
 def test_candidate_overflow_path(tmp_path, monkeypatch, stage):
     """The apply pass stages a step's solidification candidates in LDS and sends the waves that do
