@@ -568,8 +568,8 @@ __device__ __forceinline__ void nucleate_body(const DomainView &d, int p, int la
 }
 
 // Critical lengths of the liquid neighbours of frontier cell j (half-diagonal dlj): claims on
-// those reached. thr becomes the least length of the others and is returned with the claimed
-// mask. The 26 lengths are one aligned 128-byte row: the cell's own (stored by its capture) or,
+// those reached. thr becomes the least length of the others (the caller stores it) and the
+// claimed mask is returned. The 26 lengths are one aligned 128-byte row: the cell's own (stored by its capture) or,
 // for a cell-centred octahedron, its orientation's row of the seed table. They are compared all
 // at once and the claims issued after, so a lane's work does not depend on its liquid count.
 __device__ uint32_t evaluate_claims(const DomainView &d, const Grid &gr, uint32_t j, float dlj, uint32_t liquid,
@@ -606,7 +606,6 @@ __device__ uint32_t evaluate_claims(const DomainView &d, const Grid &gr, uint32_
         next = (liq && !reach) ? fminf(next, cdl[l]) : next;
     }
     for (uint32_t m = claimed; m; m &= m - 1) atomicMin(claim_word(d, j + gr.offset(__builtin_ctz(m))), j);
-    d.cells[j].thr = next;
     thr = next;
     return claimed;
 }
@@ -681,6 +680,12 @@ __device__ __forceinline__ void claim_body(const DomainView &d, int p, int layer
                 const int k = growth_index(d, w.cts, w.lid, j, cycle);
                 uint32_t liquid;
                 const int kdl = growth_base(w, k, waiting);
+                // the record's new words, written back as one sector at the end
+                unsigned long long lm = w.lm;
+                float dl = w.dl, thr = w.thr;
+                uint32_t win = w.win;
+                uint8_t ct = w.ct;
+                uint16_t flags = w.flags;
                 if (w.lm & kLmUnknown) {
                     const Coord c = decode(j, gr);
                     const uint32_t valid = valid_neighbours(c, gr.X, gr.Y, gr.nzA);
@@ -690,16 +695,15 @@ __device__ __forceinline__ void claim_body(const DomainView &d, int p, int layer
                         const uint32_t q = ((valid >> l) & 1u) ? j + gr.off[l] : j;
                         liquid |= static_cast<uint32_t>(d.cells[q].ct == kLiquid) << l;
                     }
-                    d.cells[j].lm = liquid;
+                    lm = liquid;
                     n.nmask++;
                 }
                 else {
                     liquid = static_cast<uint32_t>(w.lm & kLmNow);
                 }
-                Cell &cj = d.cells[j];
                 if (waiting) {
-                    cj.win = static_cast<uint32_t>(kdl);
-                    if (w.flags & kFresh) cj.flags = w.flags & ~kFresh;
+                    win = static_cast<uint32_t>(kdl);
+                    flags = w.flags & ~kFresh;
                     const int ahead = w.cts + 1 - cycle;
                     const uint32_t wake = ahead < 1 ? 1u
                                                     : (ahead < static_cast<int>(kAlarmSlots) - 1 ? static_cast<uint32_t>(ahead)
@@ -708,25 +712,31 @@ __device__ __forceinline__ void claim_body(const DomainView &d, int p, int layer
                 }
                 else {
                     if (!(w.flags & kCounted)) n.entered++;
-                    if ((w.flags & (kCounted | kFresh)) != kCounted) cj.flags = (w.flags & ~kFresh) | kCounted;
-                    const float dlj = replay_growth(g, w.dl, kdl, k);
-                    cj.dl = dlj;
-                    cj.win = static_cast<uint32_t>(k);
+                    flags = (w.flags & ~kFresh) | kCounted;
+                    dl = replay_growth(g, w.dl, kdl, k);
+                    win = static_cast<uint32_t>(k);
                     if (!liquid) {
-                        cj.ct = kSolid;
+                        ct = kSolid;
                         n.nsol++;
                     }
                     else {
-                        float thr = w.thr;
-                        if (dlj >= thr) {
-                            claimed = evaluate_claims(d, gr, j, dlj, liquid, w.gid, w.ct, thr);
+                        if (dl >= thr) {
+                            claimed = evaluate_claims(d, gr, j, dl, liquid, w.gid, w.ct, thr);
                             n.neval++;
                         }
                         // No neighbour left to reach (all claimed now): the captures of this step
                         // empty the mask and the settle pass decides the cell's fate.
-                        if (thr < __builtin_inff()) set_alarm(d, serial + steps_until(g, dlj, k, thr), j);
+                        if (thr < __builtin_inff()) set_alarm(d, serial + steps_until(g, dl, k, thr), j);
                     }
                 }
+                // Only this lane writes cell j in this pass (claims go to liquid cells' claim words,
+                // mask builds read liquid cells' types), so the whole sector is written back.
+                uint4 *r4 = reinterpret_cast<uint4 *>(d.cells + j);
+                r4[0] = make_uint4(static_cast<uint32_t>(lm), static_cast<uint32_t>(lm >> 32), __float_as_uint(dl),
+                                   __float_as_uint(thr));
+                r4[1] = make_uint4(static_cast<uint32_t>(w.cts), static_cast<uint32_t>(w.gid), win,
+                                   static_cast<uint32_t>(static_cast<uint8_t>(w.lid)) | (static_cast<uint32_t>(ct) << 8) |
+                                       (static_cast<uint32_t>(flags) << 16));
             }
         }
         const uint32_t slot = begin + wave_append(claimed != 0, &nclaim);
@@ -756,23 +766,38 @@ __device__ __forceinline__ void win_body(const DomainView &d, int p, uint32_t b,
     const uint32_t n = d.slice_claims[b];
     const Grid gr = grid_of(d);
     if (threadIdx.x == 0) nwin = 0;
-    for (uint32_t k0 = 0; k0 < n; k0 += kBlock) {
-        const uint32_t k = k0 + threadIdx.x;
-        uint32_t j = 0, won = 0;
-        if (k < n) {
-            j = d.claim_cell[beg + k];
-            const uint32_t cb = d.claim_bits[beg + k];
+    // two claimants per thread and round, their loads in flight together (a slice holds a little
+    // more than one workgroup's worth)
+    constexpr uint32_t kPer = 2;
+    for (uint32_t k0 = 0; k0 < n; k0 += kPer * kBlock) {
+        uint32_t j[kPer], cb[kPer], won[kPer];
+#pragma unroll
+        for (uint32_t e = 0; e < kPer; e++) {
+            const uint32_t k = k0 + e * kBlock + threadIdx.x;
+            j[e] = 0;
+            cb[e] = 0;
+            won[e] = 0;
+            if (k < n) {
+                j[e] = d.claim_cell[beg + k];
+                cb[e] = d.claim_bits[beg + k];
+            }
+        }
+#pragma unroll
+        for (uint32_t e = 0; e < kPer; e++) {
             uint32_t cw[26];
 #pragma unroll
-            for (int l = 0; l < 26; l++) cw[l] = ((cb >> l) & 1u) ? *claim_word(d, j + gr.off[l]) : kClaimNone;
+            for (int l = 0; l < 26; l++) cw[l] = ((cb[e] >> l) & 1u) ? *claim_word(d, j[e] + gr.off[l]) : kClaimNone;
 #pragma unroll
-            for (int l = 0; l < 26; l++) won |= static_cast<uint32_t>(cw[l] == j) << l;
+            for (int l = 0; l < 26; l++) won[e] |= static_cast<uint32_t>(cw[l] == j[e]) << l;
         }
         __syncthreads();
-        const uint32_t slot = wave_append(won != 0, &nwin);
-        if (won) {
-            d.claim_cell[beg + slot] = j;
-            d.claim_bits[beg + slot] = won;
+#pragma unroll
+        for (uint32_t e = 0; e < kPer; e++) {
+            const uint32_t slot = wave_append(won[e] != 0, &nwin);
+            if (won[e]) {
+                d.claim_cell[beg + slot] = j[e];
+                d.claim_bits[beg + slot] = won[e];
+            }
         }
     }
     __syncthreads();
@@ -794,8 +819,6 @@ __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t 
     __shared__ uint32_t wave_tot[kBlock / 64];
     __shared__ uint32_t pre[kBlock + 1];
     __shared__ uint32_t won_cell[kBlock], won_bits[kBlock];
-    __shared__ int32_t won_gid[kBlock];
-    __shared__ float won_cx[kBlock], won_cy[kBlock], won_cz[kBlock];
     __shared__ uint32_t cstage[kCandStage];
     __shared__ uint32_t ncstage, stage_end;
     StepCtl *ctl = d.ctl;
@@ -826,8 +849,7 @@ __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t 
     unsigned long long ncap = 0;
     uint32_t qlo = 0xFFFFFFFFu, qhi = 0;
     for (uint32_t k0 = 0; k0 < n; k0 += kBlock) {
-        // This round's winners, their grain and centre (float, as the capture uses it) in LDS, and
-        // the block-wide exclusive prefix of their capture counts.
+        // This round's winners in LDS, and the block-wide exclusive prefix of their capture counts.
         const uint32_t k = k0 + threadIdx.x;
         uint32_t j = 0, won = 0;
         if (k < n) {
@@ -837,11 +859,6 @@ __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t 
                 if (k >= upto[g - 1]) i = beg[g] + (k - upto[g - 1]);
             j = d.claim_cell[i];
             won = d.claim_bits[i];
-            const Octa &e = d.octa[j];
-            won_gid[threadIdx.x] = d.cells[j].gid;
-            won_cx[threadIdx.x] = static_cast<float>(e.x);
-            won_cy[threadIdx.x] = static_cast<float>(e.y);
-            won_cz[threadIdx.x] = static_cast<float>(e.z);
         }
         const uint32_t cnt = __builtin_popcount(won);
         uint32_t wtot;
@@ -876,8 +893,11 @@ __device__ __forceinline__ void apply_body(const DomainView &d, int p, uint32_t 
                 for (uint32_t r = t - pre[lo]; r; r--) bits &= bits - 1;
                 const int l = __builtin_ctz(bits);
                 q = jc + gr.offset(l);
-                const int h = won_gid[lo];
-                const float wcx = won_cx[lo], wcy = won_cy[lo], wcz = won_cz[lo];
+                // the winner's grain and centre (float, as the capture uses it), loaded with the
+                // neighbours' words below
+                const int h = d.cells[jc].gid;
+                const Octa &e = d.octa[jc];
+                const float wcx = static_cast<float>(e.x), wcy = static_cast<float>(e.y), wcz = static_cast<float>(e.z);
                 const Coord c = decode(jc, gr);
                 const Coord cqc{c.x + nb_dx(l), c.y + nb_dy(l), c.z + nb_dz(l)};
                 const uint32_t valid = valid_neighbours(cqc, gr.X, gr.Y, gr.nzA);
