@@ -199,6 +199,7 @@ void build_device_state(exaca_state *h) {
     const long long cap = std::min<long long>(2048, ((prop.multiProcessorCount * 4 + 7) / 8) * 8);
     const long long want = ((hd.ncell / 4096 + 7) / 8) * 8;
     h->blocks = static_cast<int>(std::max<long long>(8, std::min(cap, want)));
+    if (const int forced = exaca::frontier_blocks_override()) h->blocks = forced;
 
     DomainView &d = h->dv;
     d.X = hd.X;

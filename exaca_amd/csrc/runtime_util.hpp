@@ -39,6 +39,17 @@ inline std::string device_error_text(int flags) {
 // Candidates staged in LDS per apply workgroup: kCandStage, or fewer when EXACA_CAND_STAGE
 // (1 .. kCandStage) asks for it, so that tests can drive the apply pass's overflow list on small
 // domains.
+// Workgroups of the frontier kernels of a plain-path handle: EXACA_BLOCKS (8 .. 2048, a multiple
+// of 8) overrides the size-based choice. Tests narrow it so that slices hold many claimants and
+// the kernels' multi-round paths run.
+inline int frontier_blocks_override() {
+    const char *env = std::getenv("EXACA_BLOCKS");
+    if (!env) return 0;
+    const long v = std::strtol(env, nullptr, 10);
+    if (v < 8 || v > 2048 || v % 8)
+        throw std::invalid_argument("EXACA_BLOCKS must be a multiple of 8 in 8 .. 2048");
+    return static_cast<int>(v);
+}
 inline uint32_t cand_stage_cap() {
     const char *env = std::getenv("EXACA_CAND_STAGE");
     if (!env) return kCandStage;

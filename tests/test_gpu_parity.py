@@ -149,6 +149,34 @@ def test_quiet_steps_leave_nucleation_out(tmp_path, monkeypatch):
     assert c0 == ocycle and q0["nn"] == d["nn"] and q0["captures"] == d["captures"]
 
 
+FEW_BLOCK_CASES = {
+    "dirs_40_dense_nuclei": CASES["dirs_40_dense_nuclei"],
+    "spots_2x2": CASES["spots_2x2"],
+    # every bottom cell a substrate grain and dense nuclei: thousands of fronts at once
+    "dirs_72_dense": lambda p: c_deck(p / "f.txt", 72, density=1000, fraction=1.0),
+}
+
+
+@pytest.mark.parametrize("case", list(FEW_BLOCK_CASES))
+def test_few_workgroups_many_rounds(tmp_path, monkeypatch, case):
+    """With 8 frontier workgroups every slice holds hundreds of due cells and claimants, so the
+    capture pass, the winners pass (two claimants per thread and round, compacted in place) and the
+    apply pass all run several rounds per workgroup, and the list rebuild runs several units per
+    workgroup. A full run must still equal the oracle."""
+    monkeypatch.setenv("EXACA_BLOCKS", "8")
+    deck = FEW_BLOCK_CASES[case](tmp_path)
+    osim = orc.OracleSim(deck, DATA)
+    ocycle, _ = osim.run()
+    with ea.Solver(deck) as sol:
+        gcycle, _ = sol.run()
+        q = sol.query()
+        d = osim.dims()
+        assert gcycle == ocycle
+        for key in ("nn", "captures", "solidified", "frontier"):
+            assert q[key] == d[key], (key, q[key], d[key])
+        _compare_state(sol, osim, f"{case} on 8 workgroups")
+
+
 @pytest.mark.parametrize("stage", [1, 3])
 def test_candidate_overflow_path(tmp_path, monkeypatch, stage):
     """The apply pass stages a step's solidification candidates in LDS and sends the waves that do
