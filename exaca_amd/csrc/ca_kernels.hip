@@ -567,13 +567,34 @@ __device__ __forceinline__ void nucleate_body(const DomainView &d, int p, int la
     if (threadIdx.x == 0 && smin != 0x7FFFFFFF) atomicMin(&ctl->next_nuc[1 - p], smin);
 }
 
+// The note a frontier cell keeps in bits 2-15 of its flags after an evaluation: bits 2-6 hold
+// 1 + the neighbour whose length set thr (0: none), bits 7-15 a count d of 1/256 steps such that
+// thr + d/256 is at most the next-least length of its liquid neighbours not reached then (0: no
+// note). Liquid sets only shrink and the lengths are fixed, so when that neighbour has left the
+// mask every liquid neighbour left is at least that far: a cell woken by thr whose half-diagonal
+// is below the bound reaches none, and only its schedule changes (thr becomes the bound). Such a
+// wake-up reads no row of lengths.
+constexpr uint16_t kHintMask = 0xFFFC;
+__device__ __forceinline__ uint16_t threshold_hint(int at, float thr, float thr2) {
+    if (at < 0) return 0;
+    const double gap = (static_cast<double>(thr2) - static_cast<double>(thr)) * 256.0;  // +inf when none
+    // one 1/256 below the gap, so that the float sum below never exceeds thr2
+    const int dq = gap >= 512.0 ? 511 : static_cast<int>(gap) - 1;
+    return static_cast<uint16_t>(((at + 1) << 2) | ((dq > 0 ? dq : 0) << 7));
+}
+__device__ __forceinline__ float hint_bound(float thr, uint16_t flags) {
+    return static_cast<float>(static_cast<double>(thr) + (flags >> 7) * (1.0 / 256.0));
+}
+
 // Critical lengths of the liquid neighbours of frontier cell j (half-diagonal dlj): claims on
-// those reached. thr becomes the least length of the others (the caller stores it) and the
-// claimed mask is returned. The 26 lengths are one aligned 128-byte row: the cell's own (stored by its capture) or,
-// for a cell-centred octahedron, its orientation's row of the seed table. They are compared all
-// at once and the claims issued after, so a lane's work does not depend on its liquid count.
+// those reached. thr becomes the least length of the others (the caller stores it), hint the
+// record's note of which neighbour that is and of the next-least length (threshold_hint), and
+// the claimed mask is returned. The 26 lengths are one aligned 128-byte row: the cell's own
+// (stored by its capture) or, for a cell-centred octahedron, its orientation's row of the seed
+// table. They are compared all at once and the claims issued after, so a lane's work does not
+// depend on its liquid count.
 __device__ uint32_t evaluate_claims(const DomainView &d, const Grid &gr, uint32_t j, float dlj, uint32_t liquid,
-                                    int32_t gid, uint8_t ct, float &thr) {
+                                    int32_t gid, uint8_t ct, float &thr, uint16_t &hint) {
     const float *row = d.octa[j].cdl;
     if (ct & kSeedBit) {
         int o = orientation_of(gid);
@@ -597,16 +618,22 @@ __device__ uint32_t evaluate_claims(const DomainView &d, const Grid &gr, uint32_
     cdl[24] = v2.x;
     cdl[25] = v2.y;
     uint32_t claimed = 0;
-    float next = __builtin_inff();
+    float next = __builtin_inff(), next2 = __builtin_inff();
+    int at = -1;
 #pragma unroll
     for (int l = 0; l < 26; l++) {
         const bool liq = (liquid >> l) & 1u;
         const bool reach = dlj >= cdl[l];
         claimed |= static_cast<uint32_t>(liq && reach) << l;
-        next = (liq && !reach) ? fminf(next, cdl[l]) : next;
+        const bool open = liq && !reach;
+        const bool lower = open && cdl[l] < next;
+        next2 = lower ? next : ((open && cdl[l] < next2) ? cdl[l] : next2);
+        at = lower ? l : at;
+        next = lower ? cdl[l] : next;
     }
     for (uint32_t m = claimed; m; m &= m - 1) atomicMin(claim_word(d, j + gr.offset(__builtin_ctz(m))), j);
     thr = next;
+    hint = threshold_hint(at, next, next2);
     return claimed;
 }
 
@@ -721,8 +748,19 @@ __device__ __forceinline__ void claim_body(const DomainView &d, int p, int layer
                     }
                     else {
                         if (dl >= thr) {
-                            claimed = evaluate_claims(d, gr, j, dl, liquid, w.gid, w.ct, thr);
-                            n.neval++;
+                            const uint32_t at = (w.flags >> 2) & 31u;
+                            const float bound = hint_bound(thr, w.flags);
+                            if (at && !((liquid >> (at - 1)) & 1u) && dl < bound) {
+                                // the neighbour that set thr is gone and no other is reached
+                                thr = bound;
+                                flags &= ~kHintMask;
+                            }
+                            else {
+                                uint16_t hint = 0;
+                                claimed = evaluate_claims(d, gr, j, dl, liquid, w.gid, w.ct, thr, hint);
+                                flags = (flags & ~kHintMask) | hint;
+                                n.neval++;
+                            }
                         }
                         // No neighbour left to reach (all claimed now): the captures of this step
                         // empty the mask and the settle pass decides the cell's fate.
