@@ -38,6 +38,7 @@ _I64P = ctypes.POINTER(ctypes.c_int64)
 _SIGNATURES = {
     "exaca_create": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
     "exaca_nucleation": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "exaca_nucleation_nn": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int32)]),
     "exaca_cell_capture": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "exaca_check": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.c_int,
                                    ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_int), ctypes.c_int]),
@@ -281,6 +282,12 @@ class Solver:
 
     def nucleation(self, cycle: int, layer: int = 0) -> None:
         self._ok(self._lib.exaca_nucleation(self._h, cycle, layer))
+
+    def nucleation_nn(self, cycle: int, layer: int = 0) -> int:
+        """Nucleation returning the reference's nn (the run's nucleation count so far)."""
+        nn = ctypes.c_int32(0)
+        self._ok(self._lib.exaca_nucleation_nn(self._h, cycle, layer, ctypes.byref(nn)))
+        return nn.value
 
     def cell_capture(self, cycle: int, layer: int = 0) -> None:
         self._ok(self._lib.exaca_cell_capture(self._h, cycle, layer))

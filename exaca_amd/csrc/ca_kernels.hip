@@ -1826,8 +1826,9 @@ inline int grid_for(long long n, int cap) {
 
 }  // namespace
 
+int nucleation_blocks(const DomainView &d) { return grid_for(d.nnuclei, 1024); }
 void launch_nucleation(const DomainView &d, int parity, int layer, hipStream_t s) {
-    k_nucleate<<<grid_for(d.nnuclei, 1024), kBlock, 0, s>>>(d, parity, layer);
+    k_nucleate<<<nucleation_blocks(d), kBlock, 0, s>>>(d, parity, layer);
 }
 void launch_claim(const DomainView &d, int parity, int layer, int blocks, hipStream_t s) {
     k_claim<<<blocks, kBlock, 0, s>>>(d, parity, layer);

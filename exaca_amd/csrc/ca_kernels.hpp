@@ -12,6 +12,8 @@ constexpr int kBlock = 256;  // four wave64s per workgroup
 // steps since nucleation last ran (it would examine nothing); its capture pass is then launched
 // with nucleated = false.
 void launch_nucleation(const DomainView &d, int parity, int layer, hipStream_t s);
+// Its workgroups (each writes its own slot of the nucleation counter rows).
+int nucleation_blocks(const DomainView &d);
 // CellCapture pass A: liquidus-time gate, growth, neighbour scan, claims (src/CAupdate.cpp:217-352).
 // Also prepares the control slot of the next step.
 void launch_claim(const DomainView &d, int parity, int layer, int blocks, hipStream_t s);

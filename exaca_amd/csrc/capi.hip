@@ -719,16 +719,37 @@ int exaca_create_seeded(const char *input_file, const char *data_dir, uint64_t s
     });
 }
 
+void nucleation_call(exaca_state *h, int cycle, int layer) {
+    if (!h) throw std::invalid_argument("null handle");
+    if (h->pending >= 0) throw StateError("nucleation called twice without capture");
+    if (layer != h->layer) throw StateError("layer " + std::to_string(layer) + " is not the current layer");
+    set_next_cycle(h, cycle);
+    step_nucleation(h, layer);
+    HIP_OK(hipGetLastError());
+    h->pending = cycle;
+    h->nuc_through = cycle;
+}
+
 int exaca_nucleation(exaca_state *h, int cycle, int layer) {
+    return guarded([&] { nucleation_call(h, cycle, layer); });
+}
+
+int exaca_nucleation_nn(exaca_state *h, int cycle, int layer, int32_t *nn) {
     return guarded([&] {
-        if (!h) throw std::invalid_argument("null handle");
-        if (h->pending >= 0) throw StateError("nucleation called twice without capture");
-        if (layer != h->layer) throw StateError("layer " + std::to_string(layer) + " is not the current layer");
-        set_next_cycle(h, cycle);
-        step_nucleation(h, layer);
-        HIP_OK(hipGetLastError());
-        h->pending = cycle;
-        h->nuc_through = cycle;
+        if (!nn) throw std::invalid_argument("null nn");
+        nucleation_call(h, cycle, layer);
+        // the count booked by the steps before, plus this step's per-workgroup counts (the capture
+        // pass books them at the end of the step)
+        const int nb = exaca::nucleation_blocks(h->dv);
+        std::vector<unsigned long long> row(static_cast<size_t>(nb));
+        exaca::StepStats st;
+        HIP_OK(hipMemcpyAsync(row.data(), h->dv.part + static_cast<size_t>(exaca::kPartNucNn) * h->dv.nparts,
+                              row.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, h->stream));
+        HIP_OK(hipMemcpyAsync(&st, h->stats, sizeof st, hipMemcpyDeviceToHost, h->stream));
+        HIP_OK(hipStreamSynchronize(h->stream));
+        unsigned long long total = st.nucleated;
+        for (unsigned long long v : row) total += v;
+        *nn = static_cast<int32_t>(total);
     });
 }
 

@@ -61,6 +61,12 @@ int exaca_create_seeded(const char *input_file, const char *data_dir, uint64_t s
  * The nucleation count (the reference's nn) accumulates on the device; see exaca_query. */
 int exaca_nucleation(exaca_state *h, int cycle, int layer);
 
+/* Nucleation with the reference's `int &nn` out-parameter (src/CAupdate.hpp:15): *nn receives the
+ * run's nucleation count including this step's. Reads it back from the device, so it waits for
+ * the step's nucleation kernel, as the reference's own reduction of the count does
+ * (src/CAupdate.cpp:75-199). */
+int exaca_nucleation_nn(exaca_state *h, int cycle, int layer, int32_t *nn);
+
 /* CellCapture (src/CAupdate.hpp:21 / src/CAupdate.cpp:204) for the same cycle. Asynchronous:
  * unlike the reference (deep_copy of numSteer, src/CAupdate.cpp:243) nothing is read back. */
 int exaca_cell_capture(exaca_state *h, int cycle, int layer);

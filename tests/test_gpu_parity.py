@@ -72,16 +72,22 @@ def test_capture_geometry_bitwise(tmp_path):
 
 
 def test_stepwise_small_domain(tmp_path):
-    """Every one of the first 400 steps of a dense-nucleation 14^3 domain, state compared each step."""
+    """Every one of the first 400 steps of a dense-nucleation 14^3 domain, state compared each step;
+    the reference's nn after every step's nucleation (exaca_nucleation_nn) against the oracle's."""
     deck = c_deck(tmp_path / "s.txt", 14, density=3000, fraction=0.05)
     osim = orc.OracleSim(deck, DATA)
     with ea.Solver(deck) as sol:
         assert sol.query()["nuclei"] == osim.dims()["nuclei"] > 0
+        nn_seen = set()
         for cycle in range(1, 401):
-            sol.step(cycle)
+            nn = sol.nucleation_nn(cycle)
+            sol.cell_capture(cycle)
             osim.step(cycle)
+            assert nn == osim.dims()["nn"], f"cycle {cycle}: nn {nn} != {osim.dims()['nn']}"
+            nn_seen.add(nn)
             if cycle % 20 == 0 or cycle < 30:
                 _compare_state(sol, osim, f"cycle {cycle}")
+        assert len(nn_seen) > 3, "the deck must nucleate over several steps"
         q = sol.query()
         d = osim.dims()
         assert q["nn"] == d["nn"]
