@@ -67,17 +67,24 @@ def test_800_full_size_conservation_and_determinism():
 
 # The oracle cannot step the 800^3 deck from its start in test time, but it can continue the GPU's
 # own state: at cycle 6000 (the middle of the run, a fully developed front of 3.4 M active cells)
-# the GPU's cell types, GrainIDs, half-diagonals and double octahedron centres are imported into
-# the oracle (which derives the undercooling and the critical lengths as the reference holds them)
-# and both paths step 12 more cycles. Every cell's CellType and GrainID, and the active cells'
-# DiagonalLength and DOCenter, must then agree bit for bit: the late-front regime at full size.
+# and at cycle 9200 (the front at the domain's top plane, the run winding down) the GPU's cell
+# types, GrainIDs, half-diagonals and double octahedron centres are imported into the oracle
+# (which derives the undercooling and the critical lengths as the reference holds them) and both
+# paths step 12 more cycles. Every cell's CellType and GrainID, and the active cells'
+# DiagonalLength and DOCenter, must then agree bit for bit.
 import oracle as orc  # noqa: E402
 from conftest import DATA  # noqa: E402
 
-START, MORE = 6000, 12
+MORE = 12
+# (first cycle, least active cells then, least captures in the window): the steady front in the
+# middle of the run, and the front reaching the domain's top plane (it gets there around cycle
+# 8500-9000; the run ends at cycle 11 000)
+WINDOWS = {"steady_front_6000": (6000, 1_000_000, 500_000), "front_at_top_9200": (9200, 100_000, 50_000)}
 
 
-def test_800_front_continues_exactly_on_the_oracle():
+@pytest.mark.parametrize("window", list(WINDOWS))
+def test_800_front_continues_exactly_on_the_oracle(window):
+    START, min_active, min_captures = WINDOWS[window]
     print(f"[800^3] GPU to cycle {START}", flush=True)
     with ea.Solver(DECK) as sol:
         sol.run(max_cycles=START)
@@ -90,7 +97,7 @@ def test_800_front_continues_exactly_on_the_oracle():
         dl_g = sol.field(ea.FIELD_DIAGONAL)
         doc_g = sol.field(ea.FIELD_DOCENTER).reshape(-1, 3)
     active0 = int(np.count_nonzero((st["ct_raw"] & 0x0F) == ACTIVE))
-    assert active0 > 1_000_000
+    assert active0 > min_active
     print(f"[800^3] oracle import of {active0} active cells", flush=True)
     o = orc.OracleSim.from_state(DECK, DATA, START, st["ct_raw"], st["gid"], st["dl"], st["centre"], st["nloc"],
                                  st["ntime"])
@@ -105,7 +112,7 @@ def test_800_front_continues_exactly_on_the_oracle():
     act = np.flatnonzero(ct_o == ACTIVE)
     assert np.array_equal(dl_g[act].view(np.uint32), o.field(orc.DIAGLEN)[act].view(np.uint32))
     assert np.array_equal(doc_g[act].view(np.uint32), o.field(orc.DOCENTER).reshape(-1, 3)[act].view(np.uint32))
-    assert d["captures"] > 500_000, "the window should hold the steady front's captures"
+    assert d["captures"] > min_captures, "the window should hold the front's captures"
     print(f"[800^3] {d['captures']} captures, {d['solidified']} solidifications, {d['frontier']} active-cell updates "
           f"identical", flush=True)
     o.close()
