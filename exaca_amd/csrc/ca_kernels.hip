@@ -1,11 +1,13 @@
 // ca_kernels.hip -- the CA time step as memory-bound CDNA4 kernels over the cells due this step.
 //
-// One step (reference: src/runCA.cpp:291-311) is five launches on one stream:
+// One step (reference: src/runCA.cpp:291-311) is six launches on one stream (five when no
+// nucleation site can be due):
 //   k_nucleate  activates the pre-sampled nuclei whose time has come (src/CAupdate.cpp:67-200)
 //   k_claim     for every cell due this step: the growth since it was last looked at, a claim on
 //               every liquid neighbour it reaches (atomicMin of its index into the neighbour's
 //               claim word), and the step at which it is due next; a cell past its liquidus time
 //               with no liquid neighbour turns solid
+//   k_win       the claimants that won a claim (their index is in the claimed cell's word)
 //   k_apply     the winner of each claimed cell builds the new decentred octahedron, stores its
 //               26 critical lengths, and takes the cell out of its neighbours' liquid masks
 //   k_settle    a cell whose liquid neighbours were all taken by lower-indexed cells turns solid
