@@ -101,6 +101,9 @@ CASES = {
     "dirs_100_baseline": lambda p: EXAMPLES / "Inp_DirSolidification_100.txt",
     "dirs_40_dense_nuclei": lambda p: c_deck(p / "d.txt", 40, density=400, fraction=0.02),
     "dirs_ragged_37x23x31": lambda p: c_deck(p / "r.txt", 31, nx=37, ny=23, density=200, fraction=0.1),
+    # every cell on an x or y edge (the bounds mask on each neighbour scan and capture)
+    "dirs_narrow_2x3x40": lambda p: c_deck(p / "n.txt", 40, nx=2, ny=3, density=2000, fraction=0.5),
+    "dirs_slab_48x2x24": lambda p: c_deck(p / "b.txt", 24, nx=48, ny=2, density=500, fraction=0.2),
     "spots_2x2": lambda p: s_deck(p / "s.txt", nsx=2, nsy=2, offset=30, radius=20, density=60),
     "spots_1x2_jumps": lambda p: EXAMPLES / "Inp_SmallSpotMelt.txt",
     # multilayer: three layers 6 cells apart, each pool overlapping the one below (the window slides
