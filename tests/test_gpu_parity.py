@@ -104,6 +104,9 @@ CASES = {
     # every cell on an x or y edge (the bounds mask on each neighbour scan and capture)
     "dirs_narrow_2x3x40": lambda p: c_deck(p / "n.txt", 40, nx=2, ny=3, density=2000, fraction=0.5),
     "dirs_slab_48x2x24": lambda p: c_deck(p / "b.txt", 24, nx=48, ny=2, density=500, fraction=0.2),
+    # G = 1e5: 0.01 K per step, so the growth increments reach their plateau only after 21 004
+    # steps, past the 4096 entries staged in LDS (the kernels read the rest from global memory)
+    "dirs_slow_cooling_24": lambda p: c_deck(p / "g.txt", 24, density=100, fraction=0.1, G=1e5),
     "spots_2x2": lambda p: s_deck(p / "s.txt", nsx=2, nsy=2, offset=30, radius=20, density=60),
     "spots_1x2_jumps": lambda p: EXAMPLES / "Inp_SmallSpotMelt.txt",
     # multilayer: three layers 6 cells apart, each pool overlapping the one below (the window slides
