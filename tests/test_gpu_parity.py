@@ -107,6 +107,9 @@ CASES = {
     # G = 1e5: 0.01 K per step, so the growth increments reach their plateau only after 21 004
     # steps, past the 4096 entries staged in LDS (the kernels read the rest from global memory)
     "dirs_slow_cooling_24": lambda p: c_deck(p / "g.txt", 24, density=100, fraction=0.1, G=1e5),
+    # no spread in the nucleation undercooling: every site of a plane is due at the same step, and
+    # adjacent nuclei take each other out of shared neighbours' masks within one launch
+    "dirs_simultaneous_nuclei_20": lambda p: c_deck(p / "z.txt", 20, density=20000, fraction=0.05, dTsigma=0.0),
     "spots_2x2": lambda p: s_deck(p / "s.txt", nsx=2, nsy=2, offset=30, radius=20, density=60),
     "spots_1x2_jumps": lambda p: EXAMPLES / "Inp_SmallSpotMelt.txt",
     # multilayer: three layers 6 cells apart, each pool overlapping the one below (the window slides
