@@ -27,6 +27,7 @@
 
 #include "ca_kernels.hpp"
 #include "growth.hpp"
+#include "threshold.hpp"
 
 namespace exaca {
 namespace {
@@ -567,25 +568,6 @@ __device__ __forceinline__ void nucleate_body(const DomainView &d, int p, int la
     unsigned long long v4[4] = {nn, act, n.nsol, n.entered};
     put_parts4(d, {kPartNucNn, kPartNucAct, kPartNucSol, kPartNucEntered}, v4, b, sh);
     if (threadIdx.x == 0 && smin != 0x7FFFFFFF) atomicMin(&ctl->next_nuc[1 - p], smin);
-}
-
-// The note a frontier cell keeps in bits 2-15 of its flags after an evaluation: bits 2-6 hold
-// 1 + the neighbour whose length set thr (0: none), bits 7-15 a count d of 1/256 steps such that
-// thr + d/256 is at most the next-least length of its liquid neighbours not reached then (0: no
-// note). Liquid sets only shrink and the lengths are fixed, so when that neighbour has left the
-// mask every liquid neighbour left is at least that far: a cell woken by thr whose half-diagonal
-// is below the bound reaches none, and only its schedule changes (thr becomes the bound). Such a
-// wake-up reads no row of lengths.
-constexpr uint16_t kHintMask = 0xFFFC;
-__device__ __forceinline__ uint16_t threshold_hint(int at, float thr, float thr2) {
-    if (at < 0) return 0;
-    const double gap = (static_cast<double>(thr2) - static_cast<double>(thr)) * 256.0;  // +inf when none
-    // one 1/256 below the gap, so that the float sum below never exceeds thr2
-    const int dq = gap >= 512.0 ? 511 : static_cast<int>(gap) - 1;
-    return static_cast<uint16_t>(((at + 1) << 2) | ((dq > 0 ? dq : 0) << 7));
-}
-__device__ __forceinline__ float hint_bound(float thr, uint16_t flags) {
-    return static_cast<float>(static_cast<double>(thr) + (flags >> 7) * (1.0 / 256.0));
 }
 
 // Critical lengths of the liquid neighbours of frontier cell j (half-diagonal dlj): claims on
